@@ -1,0 +1,919 @@
+/*
+ * oracle.c -- CPU restatement of the vatai/mpk "mpk2" matrix powers kernel
+ * (MPK) path, used ONLY as test infrastructure: the pytest suite, the
+ * package smoke check and the CPU-baseline leg of bench.py call it as the
+ * checker.  Nothing in the shipped library links or calls this file.
+ *
+ * Every function follows one reference routine; the citation is given in
+ * front of it (paths relative to the reference checkout).  The code is a
+ * plain, deliberately unoptimised restatement: dense tables, fixpoint
+ * iterations and sequential loops, sized for the small parity cases only.
+ *
+ * Pinning (what checks each function, so that a dropped term or a swapped
+ * index fails a test):
+ *   orc_gen2            byte-equal to the compiled reference gen2 output
+ *   orc_comp_level      byte-equal to reference `comp level` l0 files;
+ *                       closed form for slab partitions (distance - 1)
+ *   orc_update_level    byte-equal to reference `comp level` l1/l2 files
+ *   orc_comp_skirt      byte-equal to reference `skirt` s-files; brute
+ *                       force BFS distance on tiny graphs
+ *   orc_lapjv           optimal trace vs brute force over all permutations
+ *                       (reference lab/lapjv_test.cc); permutation itself
+ *                       byte-equal through the reference rank%d.bufs files
+ *   orc_fill_buffers    byte-equal to reference mpi2_mpkwrtbufs rank%d.bufs
+ *   orc_exec_bufs       bitwise equal to reference mpi2_mpkexecbufs_val
+ *                       results (harness with seeded x) and to orc_spmv_k
+ *   orc_spmv_k          bitwise equal to reference verify_val gold_result.bin
+ *   orc_prep_tasks /
+ *   orc_vexec_tasks     bitwise equal to orc_spmv_k (shared-memory path has
+ *                       no reference output file to compare with: the
+ *                       reference mpkrun reads stale file names)
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <limits.h>
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ */
+/* PRNG shared by the whole project: SplitMix64 (Steele, Lea, Flood,   */
+/* "Fast splittable pseudorandom number generators", OOPSLA 2014).     */
+/* x_i = 2 * ((z >> 11) * 2^-53) - 1 with z the i-th SplitMix64 output.*/
+/* ------------------------------------------------------------------ */
+static uint64_t orc_splitmix_next(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+ORC_API void orc_uniform_pm1(uint64_t seed, long n, double *x) {
+  uint64_t s = seed;
+  for (long i = 0; i < n; i++) {
+    uint64_t z = orc_splitmix_next(&s);
+    x[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Generators: mpk2/gen2.c:59-268 (row order, column order and values) */
+/* ------------------------------------------------------------------ */
+static int emit(int *col, double *val, int p, int c, double v) {
+  if (col) col[p] = c;
+  if (val) val[p] = v;
+  return p + 1;
+}
+
+/* Returns n; fills ptr/col/val when non-NULL; *nnz gets the entry count.
+ * type: 0=m5p 1=t5p 2=m7p 3=m9p 4=cube (mpk2/gen2.c:59,90,139,180,229). */
+ORC_API int orc_gen2(int type, int N, int *ptr, int *col, double *val,
+                     long *nnz) {
+  int p = 0, r = 0;
+  if (ptr) ptr[0] = 0;
+  if (type == 4) {
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++)
+        for (int k = 0; k < N; k++) {
+          p = emit(col, val, p, (i * N + j) * N + k, -6.0);
+          if (i > 0) p = emit(col, val, p, ((i - 1) * N + j) * N + k, 1.0);
+          if (i < N - 1) p = emit(col, val, p, ((i + 1) * N + j) * N + k, 1.0);
+          if (j > 0) p = emit(col, val, p, (i * N + j - 1) * N + k, 1.0);
+          if (j < N - 1) p = emit(col, val, p, (i * N + j + 1) * N + k, 1.0);
+          if (k > 0) p = emit(col, val, p, (i * N + j) * N + k - 1, 1.0);
+          if (k < N - 1) p = emit(col, val, p, (i * N + j) * N + k + 1, 1.0);
+          if (ptr) ptr[++r] = p;
+        }
+    *nnz = p;
+    return N * N * N;
+  }
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++) {
+      int c = i * N + j;
+      if (type == 0) { /* m5p */
+        if (i > 0) p = emit(col, val, p, c - N, 1.0);
+        if (j > 0) p = emit(col, val, p, c - 1, 1.0);
+        p = emit(col, val, p, c, -4.0);
+        if (j < N - 1) p = emit(col, val, p, c + 1, 1.0);
+        if (i < N - 1) p = emit(col, val, p, c + N, 1.0);
+      } else if (type == 1) { /* t5p */
+        if (i == N - 1) p = emit(col, val, p, j, 1.0);
+        if (i > 0) p = emit(col, val, p, c - N, 1.0);
+        if (j == N - 1) p = emit(col, val, p, i * N, 1.0);
+        if (j > 0) p = emit(col, val, p, c - 1, 1.0);
+        p = emit(col, val, p, c, -4.0);
+        if (i < N - 1) p = emit(col, val, p, c + N, 1.0);
+        if (j < N - 1) p = emit(col, val, p, c + 1, 1.0);
+        if (i == 0) p = emit(col, val, p, (N - 1) * N + j, 1.0);
+        if (j == 0) p = emit(col, val, p, i * N + N - 1, 1.0);
+      } else if (type == 2) { /* m7p */
+        if (i > 0 && j > 0) p = emit(col, val, p, c - N - 1, 1.0);
+        if (i > 0) p = emit(col, val, p, c - N, 1.0);
+        if (j > 0) p = emit(col, val, p, c - 1, 1.0);
+        p = emit(col, val, p, c, -6.0);
+        if (j < N - 1) p = emit(col, val, p, c + 1, 1.0);
+        if (i < N - 1) p = emit(col, val, p, c + N, 1.0);
+        if (i < N - 1 && j < N - 1) p = emit(col, val, p, c + N + 1, 1.0);
+      } else { /* m9p */
+        if (i > 0) p = emit(col, val, p, c - N, 1.0);
+        if (i < N - 1) p = emit(col, val, p, c + N, 1.0);
+        if (j > 0) p = emit(col, val, p, c - 1, 1.0);
+        if (j < N - 1) p = emit(col, val, p, c + 1, 1.0);
+        p = emit(col, val, p, c, -8.0);
+        if (i > 0 && j > 0) p = emit(col, val, p, c - N - 1, 1.0);
+        if (i < N - 1 && j < N - 1) p = emit(col, val, p, c + N + 1, 1.0);
+        if (i > 0 && j < N - 1) p = emit(col, val, p, c - N + 1, 1.0);
+        if (i < N - 1 && j > 0) p = emit(col, val, p, c + N - 1, 1.0);
+      }
+      if (ptr) ptr[++r] = p;
+    }
+  *nnz = p;
+  return N * N;
+}
+
+/* ------------------------------------------------------------------ */
+/* Levels: mpk2/lib.c:183-199 (minplus_r), :330-371 (comp_level),      */
+/* :373-412 (update_level).  Fixpoint iteration exactly as written.    */
+/* ------------------------------------------------------------------ */
+static int orc_minplus_r(int n, const int *ptr, const int *col, const int *x,
+                         int *y, int r, const int *part) {
+  int changes = 0;
+  for (int i = 0; i < n; i++) {
+    if (part[i] != r) continue;
+    int best = x[i];
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+      int cand = x[col[j]] + 1;
+      if (cand < best) { best = cand; changes++; }
+    }
+    y[i] = best;
+  }
+  return changes;
+}
+
+static int orc_minplus_x(int n, const int *ptr, const int *col, const int *x,
+                         int *y, int r, const int *part) {
+  int changes = 0;
+  for (int i = 0; i < n; i++) {
+    if (part[i] == r) continue;
+    int best = x[i];
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+      int cand = x[col[j]] + 1;
+      if (cand < best) { best = cand; changes++; }
+    }
+    y[i] = best;
+  }
+  return changes;
+}
+
+static int orc_npart(int n, const int *part) {
+  int m = 0;
+  for (int i = 0; i < n; i++) if (part[i] > m) m = part[i];
+  return m + 1; /* mpk2/lib.c:273-284 (read_part) */
+}
+
+/* lorg == NULL: comp_level; otherwise update_level. */
+ORC_API void orc_levels(int n, const int *ptr, const int *col,
+                        const int *part, const int *lorg, int *level) {
+  int np = orc_npart(n, part);
+  int *a = malloc(sizeof(int) * n), *b = malloc(sizeof(int) * n);
+  int big = n + 1;
+  if (lorg) {
+    int mx = lorg[0];
+    for (int i = 1; i < n; i++) if (lorg[i] > mx) mx = lorg[i];
+    big = mx + n + 1;
+  }
+  for (int r = 0; r < np; r++) {
+    for (int i = 0; i < n; i++) {
+      int v = (part[i] == r) ? big : (lorg ? lorg[i] : -1);
+      a[i] = b[i] = v;
+    }
+    int ch;
+    do {
+      ch = orc_minplus_r(n, ptr, col, a, b, r, part);
+      int *t = a; a = b; b = t;
+    } while (ch > 0);
+    for (int i = 0; i < n; i++) if (part[i] == r) level[i] = b[i];
+  }
+  free(a); free(b);
+}
+
+/* comp_skirt, mpk2/lib.c:584-615: out[p*n+j] = BFS distance of j from
+ * partition p (n+1 when unreachable). */
+ORC_API void orc_comp_skirt(int n, const int *ptr, const int *col,
+                            const int *part, int *out) {
+  int np = orc_npart(n, part);
+  int *a = malloc(sizeof(int) * n), *b = malloc(sizeof(int) * n);
+  for (int p = 0; p < np; p++) {
+    for (int i = 0; i < n; i++) a[i] = b[i] = (part[i] == p) ? 0 : n + 1;
+    int ch;
+    do {
+      ch = orc_minplus_x(n, ptr, col, a, b, p, part);
+      int *t = a; a = b; b = t;
+    } while (ch > 0);
+    for (int i = 0; i < n; i++) out[(long)p * n + i] = b[i];
+  }
+  free(a); free(b);
+}
+
+/* write_skirt + read_skirt round trip, mpk2/lib.c:617-654 and :656-707:
+ * sl = dist when the entry is written, -1 otherwise.  lastl == NULL is
+ * the nphase == 0 case. */
+ORC_API void orc_skirt_filter(int n, int np, int nlevel, const int *dist,
+                              const int *lastl, int *sl) {
+  for (int p = 0; p < np; p++)
+    for (int j = 0; j < n; j++) {
+      int d = dist[(long)p * n + j];
+      int w = lastl ? (lastl[j] < nlevel - d) : (d < nlevel);
+      sl[(long)p * n + j] = w ? d : -1;
+    }
+}
+
+/* Edge weights for the next partitioning step, mpk2/lib.c:504-537. */
+ORC_API void orc_level2wcrs(int n, const int *ptr, const int *col,
+                            const int *level, int *we) {
+  int mn = level[0];
+  for (int i = 1; i < n; i++) if (level[i] < mn) mn = level[i];
+  for (int i = 0; i < n; i++)
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+      double w = level[i] + level[col[j]] - 2 * mn;
+      w = 1e+6 / (w + 1);
+      we[j] = (w < 1.0) ? 1 : (int)w;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Linear assignment, mpk2/lapjv.c:141-390 (Jonker & Volgenant 1987,   */
+/* the reference's variant: column reduction, reduction transfer, two  */
+/* augmenting row reductions, shortest augmenting paths).  The cost is */
+/* max(sums) - sums so that the trace of sums is maximised.            */
+/* ------------------------------------------------------------------ */
+#define UNASG (-1)
+static int mark_(int v) { return v < UNASG ? v : -v - 2; }
+static int unmark_(int v) { return -v - 2; }
+
+ORC_API void orc_lapjv(const int *sums, int np, int *perm) {
+  int *c = malloc(sizeof(int) * np * np);
+  int *x = malloc(sizeof(int) * np);   /* column of row   */
+  int *y = malloc(sizeof(int) * np);   /* row of column   */
+  int *v = malloc(sizeof(int) * np);   /* column duals    */
+  int *cl = malloc(sizeof(int) * np);  /* column list     */
+  int *fr = malloc(sizeof(int) * np);  /* free rows       */
+  int *d = malloc(sizeof(int) * np), *pred = malloc(sizeof(int) * np);
+  int nfree = 0;
+  int mx = sums[0];
+  for (int i = 1; i < np * np; i++) if (sums[i] > mx) mx = sums[i];
+  for (int i = 0; i < np * np; i++) c[i] = mx - sums[i];
+  for (int i = 0; i < np; i++) { x[i] = UNASG; y[i] = UNASG; v[i] = 0; }
+  /* column reduction (lapjv.c:176-198) */
+  for (int j = np - 1; j >= 0; j--) {
+    cl[j] = j;
+    int mn = c[j], imin = 0;
+    for (int i = 1; i < np; i++)
+      if (c[i * np + j] < mn) { mn = c[i * np + j]; imin = i; }
+    v[j] = mn;
+    if (x[imin] == UNASG) { x[imin] = j; y[j] = imin; }
+    else { x[imin] = mark_(x[imin]); y[j] = UNASG; }
+  }
+  /* reduction transfer (lapjv.c:200-221) */
+  for (int i = 0; i < np; i++) {
+    if (x[i] == UNASG) fr[nfree++] = i;
+    else if (x[i] < UNASG) x[i] = unmark_(x[i]);
+    else {
+      int j1 = x[i], mn = INT_MAX;
+      for (int j = 0; j < np; j++)
+        if (j != j1 && c[i * np + j] - v[j] < mn) mn = c[i * np + j] - v[j];
+      v[j1] -= mn;
+    }
+  }
+  /* augmenting row reduction, twice (lapjv.c:223-265) */
+  for (int pass = 0; pass < 2; pass++) {
+    int total = nfree, k = 0;
+    nfree = 0;
+    while (k < total) {
+      int i = fr[k++];
+      int u1 = c[i * np] - v[0], u2 = INT_MAX, j1 = 0, j2 = 0;
+      for (int j = 1; j < np; j++) {
+        int h = c[i * np + j] - v[j];
+        if (h < u2) {
+          if (h >= u1) { u2 = h; j2 = j; }
+          else { u2 = u1; u1 = h; j2 = j1; j1 = j; }
+        }
+      }
+      int i1 = y[j1];
+      if (u1 < u2) v[j1] = v[j1] - u2 + u1;
+      else if (i1 > UNASG) { j1 = j2; i1 = y[j1]; }
+      if (i1 > UNASG) {
+        if (u1 < u2) fr[--k] = i1;
+        else fr[nfree++] = i1;
+      }
+      x[i] = j1;
+      y[j1] = i;
+    }
+  }
+  /* augmentation (lapjv.c:267-353) */
+  int total = nfree;
+  for (int f = 0; f < total; f++) {
+    int i1 = fr[f];
+    int low = 0, up = 0, last = 0, mn = 0, jend = -1;
+    for (int j = 0; j < np; j++) { d[j] = c[i1 * np + j] - v[j]; pred[j] = i1; }
+    for (;;) {
+      if (up == low) {
+        last = low;
+        mn = d[cl[up++]];
+        for (int k = up; k < np; k++) {
+          int j = cl[k], h = d[j];
+          if (h <= mn) {
+            if (h < mn) { up = low; mn = h; }
+            cl[k] = cl[up];
+            cl[up++] = j;
+          }
+        }
+        for (int h = low; h < up; h++)
+          if (y[cl[h]] == UNASG) { jend = cl[h]; break; }
+        if (jend >= 0) break;
+      }
+      int j1 = cl[low++];
+      int i = y[j1];
+      int u1 = c[i * np + j1] - v[j1] - mn;
+      for (int k = up; k < np; k++) {
+        int j = cl[k];
+        int h = c[i * np + j] - v[j] - u1;
+        if (h < d[j]) {
+          d[j] = h;
+          pred[j] = i;
+          if (h == mn) {
+            if (y[j] == UNASG) { jend = j; break; }
+            cl[k] = cl[up];
+            cl[up++] = j;
+          }
+        }
+      }
+      if (jend >= 0) break;
+    }
+    for (int k = 0; k < last; k++) {
+      int j1 = cl[k];
+      v[j1] = v[j1] + d[j1] - mn;
+    }
+    int i, j = jend;
+    do {
+      i = pred[j];
+      y[j] = i;
+      int kk = j;
+      j = x[i];
+      x[i] = kk;
+    } while (i != i1);
+  }
+  for (int j = 0; j < np; j++) perm[j] = y[j];
+  free(c); free(x); free(y); free(v); free(cl); free(fr); free(d); free(pred);
+}
+
+/* ------------------------------------------------------------------ */
+/* Buffers: mpk2/buffers.c:8-474 + reduce_comm.c:31-82 +               */
+/* mpi2_mpkwrtbufs.c:12-30.  Dense communication table exactly as the  */
+/* reference keeps it (char[nlevel*n*np*np]); small sizes only.        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int n, np, nlevel, nphase;
+  const int *ptr, *col;
+  int **part;        /* part[ph], ph < max(nphase,1); relabelled in place */
+  const int **lev;   /* lev[ph], ph < nphase */
+  const int *sl;     /* skirt levels [np*n] after filtering */
+  char *ct;
+  int *sp;           /* store_part [n*(nlevel+1)] */
+} orc_cd;
+
+static long ct_at(const orc_cd *cd, int s, int t, long v) {
+  return (long)cd->nlevel * cd->n * ((long)cd->np * s + t) + v;
+}
+static void ct_clear(orc_cd *cd) {
+  memset(cd->ct, 0, (size_t)cd->nlevel * cd->n * cd->np * cd->np);
+}
+static void ct_init(orc_cd *cd) { /* buffers.c:45-51 */
+  for (int i = 0; i < cd->n; i++) {
+    int p = cd->part[0][i];
+    cd->ct[ct_at(cd, p, p, i)] = 1;
+  }
+}
+static int lmin_prev(const orc_cd *cd, int phm1) { /* buffers.c:68-77 */
+  if (cd->nphase == 0 || phm1 == -1) return 0;
+  const int *l = cd->lev[phm1];
+  int m = l[0];
+  for (int i = 0; i < cd->n; i++) if (l[i] < m) m = l[i];
+  return m;
+}
+static int lmax_cap(const orc_cd *cd, int ph) { /* buffers.c:79-90 */
+  if (ph == cd->nphase) return cd->nlevel;
+  const int *l = cd->lev[ph];
+  int m = l[0];
+  for (int i = 1; i < cd->n; i++) if (l[i] > m) m = l[i];
+  return m > cd->nlevel ? cd->nlevel : m;
+}
+static int prev_level(const orc_cd *cd, int ph, int i) {
+  return ph ? cd->lev[ph - 1][i] : 0;
+}
+static void mark_sources(orc_cd *cd, int tgt, int i, int level) {
+  /* buffers.c:53-66 (proc_vertex) */
+  for (int j = cd->ptr[i]; j < cd->ptr[i + 1]; j++) {
+    long sv = (long)cd->n * (level - 1) + cd->col[j];
+    int s = cd->sp[sv];
+    if (s == -1) { fprintf(stderr, "oracle: source not stored\n"); abort(); }
+    if (s != tgt) cd->ct[ct_at(cd, s, tgt, sv)] = 1;
+  }
+}
+static void collect_comm(orc_cd *cd, int ph) { /* buffers.c:92-123 */
+  ct_clear(cd);
+  int n = cd->n;
+  const int *ll = cd->lev[ph];
+  for (int level = lmin_prev(cd, ph - 1) + 1; level <= lmax_cap(cd, ph); level++)
+    for (int i = 0; i < n; i++) {
+      if (!(prev_level(cd, ph, i) < level && level <= ll[i])) continue;
+      int tgt = cd->part[ph][i];
+      for (int j = cd->ptr[i]; j < cd->ptr[i + 1]; j++) {
+        long sv = (long)n * (level - 1) + cd->col[j];
+        int s = cd->sp[sv];
+        if (s != -1) cd->ct[ct_at(cd, s, tgt, sv)] = 1;
+      }
+    }
+}
+static void relabel(orc_cd *cd, int ph) { /* reduce_comm.c:31-82 */
+  int np = cd->np;
+  long size = (long)cd->n * cd->nlevel;
+  int *sums = malloc(sizeof(int) * np * np), *perm = malloc(sizeof(int) * np);
+  for (int s = 0; s < np; s++)
+    for (int t = 0; t < np; t++) {
+      int cnt = 0;
+      for (long v = 0; v < size; v++) cnt += cd->ct[ct_at(cd, s, t, v)];
+      sums[s * np + t] = cnt;
+    }
+  orc_lapjv(sums, np, perm);
+  for (int i = 0; i < cd->n; i++) cd->part[ph][i] = perm[cd->part[ph][i]];
+  free(sums); free(perm);
+}
+static void phase_table(orc_cd *cd, int ph, int first) { /* buffers.c:125-153 */
+  int n = cd->n;
+  const int *ll = cd->lev[ph];
+  for (int level = lmin_prev(cd, ph - 1) + 1; level <= lmax_cap(cd, ph); level++)
+    for (int i = 0; i < n; i++) {
+      long iv = (long)n * level + i;
+      if (prev_level(cd, ph, i) < level && level <= ll[i] &&
+          (!first || cd->sp[iv] == -1)) {
+        int cp = cd->part[ph][i];
+        mark_sources(cd, cp, i, level);
+        if (first) cd->sp[iv] = cp;
+      }
+    }
+}
+static void skirt_table(orc_cd *cd) { /* buffers.c:155-179 */
+  int n = cd->n, nl = cd->nlevel, nph = cd->nphase;
+  for (int p = 0; p < cd->np; p++)
+    for (int level = lmin_prev(cd, nph - 1) + 1; level <= nl; level++)
+      for (int i = 0; i < n; i++) {
+        int s = cd->sl[(long)p * n + i];
+        if (prev_level(cd, nph, i) < level && s >= 0 && level <= nl - s) {
+          mark_sources(cd, p, i, level);
+          cd->sp[(long)n * level + i] = p;
+        }
+      }
+}
+static int cond_phase(const orc_cd *cd, int rank, int ph, int i, int level) {
+  /* buffers.c:194-201 */
+  int pl = prev_level(cd, ph, i);
+  int cp = ph == cd->nphase ? cd->part[0][i] : cd->part[ph][i];
+  int cl = ph == cd->nphase ? cd->sl[(long)rank * cd->n + i] : cd->lev[ph][i];
+  return pl < level && level <= cl && rank == cp;
+}
+static int cond_skirt(const orc_cd *cd, int rank, int ph, int i, int level) {
+  /* buffers.c:203-207 */
+  (void)ph;
+  int pl = cd->nphase ? cd->lev[cd->nphase - 1][i] : 0;
+  int s = cd->sl[(long)rank * cd->n + i];
+  return pl < level && 0 <= s && level <= cd->nlevel - s;
+}
+
+typedef struct {
+  int n, npart, nlevel, nphase, rank, buf_count, buf_scount, mptr_count,
+      mcol_count;
+  int *recvcounts, *sendcounts, *rdispls, *sdispls;
+  int *rcount, *mcount, *scount, *rbuf_offsets, *mbuf_offsets, *sbuf_offsets,
+      *mptr_offsets, *mcol_offsets;
+  long *idx_buf, *idx_sbuf, *mptr_buf, *mcol_buf;
+  double *mval_buf;
+} orc_bufs;
+
+static void walk_rows(const orc_cd *cd, orc_bufs *b, int ph, int skirt,
+                      int fill) {
+  /* buffers.c:218-238 (iterator; its prevlmin always evaluates to 0) */
+  int mx = lmax_cap(cd, ph);
+  long *out = fill ? b->idx_buf + b->mbuf_offsets[ph] : NULL;
+  int cnt = 0;
+  for (int level = 1; level <= mx; level++)
+    for (int i = 0; i < cd->n; i++) {
+      int ok = skirt ? cond_skirt(cd, b->rank, ph, i, level)
+                     : cond_phase(cd, b->rank, ph, i, level);
+      if (ok) {
+        if (out) out[cnt] = (long)level * cd->n + i;
+        cnt++;
+      }
+    }
+  b->mcount[ph] = cnt;
+}
+static void rs_counts(const orc_cd *cd, orc_bufs *b, int ph) {
+  /* buffers.c:246-282 */
+  int np = cd->np, r = b->rank;
+  long size = (long)cd->n * cd->nlevel;
+  b->scount[ph] = b->rcount[ph] = 0;
+  for (int p = 0; p < np; p++) {
+    int sc = 0, rc = 0;
+    for (long v = 0; v < size; v++) {
+      sc += cd->ct[ct_at(cd, r, p, v)];
+      rc += cd->ct[ct_at(cd, p, r, v)];
+    }
+    b->sendcounts[ph * np + p] = sc;
+    b->recvcounts[ph * np + p] = rc;
+    b->scount[ph] += sc;
+    b->rcount[ph] += rc;
+  }
+  for (int p = 0; p < np; p++) {
+    int k = ph * np + p;
+    b->sdispls[k] = p ? b->sdispls[k - 1] + b->sendcounts[k - 1] : 0;
+    b->rdispls[k] = p ? b->rdispls[k - 1] + b->recvcounts[k - 1] : 0;
+  }
+}
+static void rs_fill(const orc_cd *cd, orc_bufs *b, int ph) {
+  /* buffers.c:354-377 */
+  int np = cd->np, r = b->rank, sc = 0, rc = 0;
+  long size = (long)cd->n * cd->nlevel;
+  long *rb = b->idx_buf + b->rbuf_offsets[ph];
+  long *sb = b->idx_sbuf + b->sbuf_offsets[ph];
+  for (int p = 0; p < np; p++)
+    for (long v = 0; v < size; v++) {
+      if (cd->ct[ct_at(cd, r, p, v)]) sb[sc++] = v;
+      if (cd->ct[ct_at(cd, p, r, v)]) rb[rc++] = v;
+    }
+}
+static void fill_ptrs(const orc_cd *cd, orc_bufs *b, int ph) {
+  /* buffers.c:379-390 */
+  long *mp = b->mptr_buf + b->mptr_offsets[ph];
+  const long *im = b->idx_buf + b->mbuf_offsets[ph];
+  mp[0] = 0;
+  for (int m = 0; m < b->mcount[ph]; m++) {
+    int i = (int)(im[m] % cd->n);
+    mp[m + 1] = mp[m] + cd->ptr[i + 1] - cd->ptr[i];
+  }
+}
+
+static void put(unsigned char **w, const void *src, size_t bytes) {
+  memcpy(*w, src, bytes);
+  *w += bytes;
+}
+
+/* Builds every rank's buffers (mpk2/buffers.c:573-586 + mval fill of
+ * mpi2_mpkwrtbufs.c:12-30) and serialises them in the rank%d.bufs layout
+ * of write_buffers (buffers.c:588-631).  part[] arrays are relabelled in
+ * place like the reference does.  blob[r]/blob_len[r] are malloc'ed. */
+ORC_API int orc_fill_buffers(int n, const int *ptr, const int *col,
+                             const double *val, int nlevel, int nphase,
+                             int **part, const int **lev, const int *sl,
+                             unsigned char **blob, long *blob_len) {
+  int np = orc_npart(n, part[0]);
+  for (int rank = 0; rank < np; rank++) {
+    /* Every rank redoes the whole construction (the reference runs it once
+     * per MPI rank); partitions therefore must be restored per rank. */
+    int nparts = nphase > 0 ? nphase : 1;
+    int **pcopy = malloc(sizeof(int *) * nparts);
+    for (int ph = 0; ph < nparts; ph++) {
+      pcopy[ph] = malloc(sizeof(int) * n);
+      memcpy(pcopy[ph], part[ph], sizeof(int) * n);
+    }
+    orc_cd cd = {n, np, nlevel, nphase, ptr, col, pcopy, lev, sl, NULL, NULL};
+    long spn = (long)n * (nlevel + 1);
+    cd.ct = malloc((size_t)nlevel * n * np * np);
+    cd.sp = malloc(sizeof(int) * spn);
+    for (long v = 0; v < spn; v++) cd.sp[v] = v < n ? pcopy[0][v] : -1;
+
+    orc_bufs b;
+    memset(&b, 0, sizeof(b));
+    b.n = n; b.npart = np; b.nlevel = nlevel; b.nphase = nphase; b.rank = rank;
+    int a1 = np * (nphase + 1), a2 = nphase + 1;
+    b.recvcounts = calloc(a1, sizeof(int)); b.sendcounts = calloc(a1, sizeof(int));
+    b.rdispls = calloc(a1, sizeof(int)); b.sdispls = calloc(a1, sizeof(int));
+    b.rcount = calloc(a2, sizeof(int)); b.mcount = calloc(a2, sizeof(int));
+    b.scount = calloc(a2, sizeof(int)); b.rbuf_offsets = calloc(a2, sizeof(int));
+    b.mbuf_offsets = calloc(a2, sizeof(int)); b.sbuf_offsets = calloc(a2, sizeof(int));
+    b.mptr_offsets = calloc(a2, sizeof(int)); b.mcol_offsets = calloc(a2, sizeof(int));
+
+    /* pass 1: sizes (buffers.c:284-315) */
+    ct_clear(&cd);
+    ct_init(&cd);
+    for (int ph = 0; ph < nphase; ph++) {
+      if (ph > 0) {
+        collect_comm(&cd, ph);
+        relabel(&cd, ph);
+        ct_clear(&cd);
+      }
+      phase_table(&cd, ph, 1);
+      walk_rows(&cd, &b, ph, 0, 0);
+      rs_counts(&cd, &b, ph);
+      ct_clear(&cd);
+    }
+    skirt_table(&cd);
+    walk_rows(&cd, &b, nphase, 1, 0);
+    rs_counts(&cd, &b, nphase);
+
+    /* allocation and offsets (buffers.c:317-352) */
+    long cnt = 0, scnt = 0, mpc = 0;
+    for (int ph = 0; ph <= nphase; ph++) {
+      b.rbuf_offsets[ph] = cnt; cnt += b.rcount[ph];
+      b.mbuf_offsets[ph] = cnt; cnt += b.mcount[ph];
+      b.sbuf_offsets[ph] = scnt; scnt += b.scount[ph];
+      b.mptr_offsets[ph] = mpc; mpc += b.mcount[ph] + 1;
+    }
+    b.buf_count = cnt; b.buf_scount = scnt; b.mptr_count = mpc;
+    b.idx_buf = malloc(sizeof(long) * (cnt ? cnt : 1));
+    b.idx_sbuf = malloc(sizeof(long) * (scnt ? scnt : 1));
+    b.mptr_buf = malloc(sizeof(long) * mpc);
+
+    /* pass 2: contents (buffers.c:392-411) */
+    ct_clear(&cd);
+    ct_init(&cd);
+    for (int ph = 0; ph < nphase; ph++) {
+      phase_table(&cd, ph, 0);
+      walk_rows(&cd, &b, ph, 0, 1);
+      rs_fill(&cd, &b, ph);
+      ct_clear(&cd);
+      fill_ptrs(&cd, &b, ph);
+    }
+    skirt_table(&cd);
+    walk_rows(&cd, &b, nphase, 1, 1);
+    rs_fill(&cd, &b, nphase);
+    fill_ptrs(&cd, &b, nphase);
+
+    /* columns and send indices (buffers.c:413-474) */
+    long mcc = 0;
+    for (int ph = 0; ph <= nphase; ph++) {
+      b.mcol_offsets[ph] = mcc;
+      mcc += b.mptr_buf[b.mptr_offsets[ph] + b.mcount[ph]];
+    }
+    b.mcol_count = mcc;
+    b.mcol_buf = malloc(sizeof(long) * (mcc ? mcc : 1));
+    b.mval_buf = malloc(sizeof(double) * (mcc ? mcc : 1));
+    int *first = malloc(sizeof(int) * spn);
+    for (long v = 0; v < spn; v++) first[v] = -1;
+    for (int i = 0; i < b.buf_count; i++) {
+      int v = (int)b.idx_buf[i];
+      if (first[v] == -1) first[v] = i;
+    }
+    for (int ph = 0; ph <= nphase; ph++) {
+      const long *mp = b.mptr_buf + b.mptr_offsets[ph];
+      const long *im = b.idx_buf + b.mbuf_offsets[ph];
+      long *mc = b.mcol_buf + b.mcol_offsets[ph];
+      double *mv = b.mval_buf + b.mcol_offsets[ph];
+      for (int m = 0; m < b.mcount[ph]; m++) {
+        long i = im[m] % n, level = im[m] / n;
+        for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+          mc[j - ptr[i] + mp[m]] = first[col[j] + (long)n * (level - 1)];
+          mv[j - ptr[i] + mp[m]] = val[j];
+        }
+      }
+    }
+    for (int i = 0; i < b.buf_scount; i++) {
+      int v = (int)b.idx_sbuf[i];
+      b.idx_sbuf[i] = first[v];
+    }
+    free(first);
+
+    /* serialise (buffers.c:588-631) */
+    long bytes = 9 * 4 + 4L * a1 * 4 + 8L * a2 * 4 +
+                 8L * (b.buf_count + b.buf_scount + b.mptr_count + b.mcol_count) +
+                 8L * b.mcol_count;
+    unsigned char *w = malloc(bytes), *w0 = w;
+    put(&w, &b.n, 4); put(&w, &b.npart, 4); put(&w, &b.nlevel, 4);
+    put(&w, &b.nphase, 4); put(&w, &b.rank, 4); put(&w, &b.buf_count, 4);
+    put(&w, &b.buf_scount, 4); put(&w, &b.mptr_count, 4);
+    put(&w, &b.mcol_count, 4);
+    put(&w, b.recvcounts, 4L * a1); put(&w, b.sendcounts, 4L * a1);
+    put(&w, b.rdispls, 4L * a1); put(&w, b.sdispls, 4L * a1);
+    put(&w, b.rcount, 4L * a2); put(&w, b.mcount, 4L * a2);
+    put(&w, b.scount, 4L * a2); put(&w, b.rbuf_offsets, 4L * a2);
+    put(&w, b.mbuf_offsets, 4L * a2); put(&w, b.sbuf_offsets, 4L * a2);
+    put(&w, b.mptr_offsets, 4L * a2); put(&w, b.mcol_offsets, 4L * a2);
+    put(&w, b.idx_buf, 8L * b.buf_count); put(&w, b.idx_sbuf, 8L * b.buf_scount);
+    put(&w, b.mptr_buf, 8L * b.mptr_count); put(&w, b.mcol_buf, 8L * b.mcol_count);
+    put(&w, b.mval_buf, 8L * b.mcol_count);
+    blob[rank] = w0;
+    blob_len[rank] = bytes;
+
+    free(b.recvcounts); free(b.sendcounts); free(b.rdispls); free(b.sdispls);
+    free(b.rcount); free(b.mcount); free(b.scount); free(b.rbuf_offsets);
+    free(b.mbuf_offsets); free(b.sbuf_offsets); free(b.mptr_offsets);
+    free(b.mcol_offsets); free(b.idx_buf); free(b.idx_sbuf); free(b.mptr_buf);
+    free(b.mcol_buf); free(b.mval_buf);
+    free(cd.ct); free(cd.sp);
+    if (rank == np - 1)
+      for (int ph = 0; ph < nparts; ph++) memcpy(part[ph], pcopy[ph], sizeof(int) * n);
+    for (int ph = 0; ph < nparts; ph++) free(pcopy[ph]);
+    free(pcopy);
+  }
+  return np;
+}
+
+ORC_API void orc_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------ */
+/* Exec of the buffers of all ranks in one process:                    */
+/* mpk2/mpi2_mpkexecbufs_val.c:13-56 (do_task, do_comm, phase driver), */
+/* MPI_Alltoallv restated as a copy from every sender's send buffer,   */
+/* and collect_results, mpk2/mpi2_mpkexec.c:122-146.                   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int hdr[9];
+  const int *rc, *sc, *rd, *sd;
+  const int *rcount, *mcount, *scount, *roff, *moff, *soff, *mpoff, *mcoff;
+  const long *idx, *sidx, *mptr, *mcol;
+  const double *mval;
+  double *vv, *sbuf;
+} orc_view;
+
+static void view_of(const unsigned char *blob, orc_view *v) {
+  const unsigned char *r = blob;
+  memcpy(v->hdr, r, 36); r += 36;
+  int np = v->hdr[1], nph = v->hdr[3];
+  int a1 = np * (nph + 1), a2 = nph + 1;
+  v->rc = (const int *)r; r += 4L * a1;
+  v->sc = (const int *)r; r += 4L * a1;
+  v->rd = (const int *)r; r += 4L * a1;
+  v->sd = (const int *)r; r += 4L * a1;
+  v->rcount = (const int *)r; r += 4L * a2;
+  v->mcount = (const int *)r; r += 4L * a2;
+  v->scount = (const int *)r; r += 4L * a2;
+  v->roff = (const int *)r; r += 4L * a2;
+  v->moff = (const int *)r; r += 4L * a2;
+  v->soff = (const int *)r; r += 4L * a2;
+  v->mpoff = (const int *)r; r += 4L * a2;
+  v->mcoff = (const int *)r; r += 4L * a2;
+  v->idx = (const long *)r; r += 8L * v->hdr[5];
+  v->sidx = (const long *)r; r += 8L * v->hdr[6];
+  v->mptr = (const long *)r; r += 8L * v->hdr[7];
+  v->mcol = (const long *)r; r += 8L * v->hdr[8];
+  v->mval = (const double *)r;
+}
+
+static void task(orc_view *v, int ph) {
+  const long *mp = v->mptr + v->mpoff[ph];
+  const long *mc = v->mcol + v->mcoff[ph];
+  const double *mv = v->mval + v->mcoff[ph];
+  double *out = v->vv + v->moff[ph];
+  for (int m = 0; m < v->mcount[ph]; m++) {
+    double t = 0.0;
+    for (long j = mp[m]; j < mp[m + 1]; j++) t += mv[j] * v->vv[mc[j]];
+    out[m] = t;
+  }
+}
+
+/* x[n] input (the reference harness fills vv_buf[i] = x[idx_buf[i]] for
+ * i < rcount[0]); vv[n*(nlevel+1)] output, level-major. */
+ORC_API void orc_exec_bufs(int np, const unsigned char **blobs, const double *x,
+                           double *vv) {
+  orc_view *v = calloc(np, sizeof(orc_view));
+  for (int p = 0; p < np; p++) {
+    view_of(blobs[p], &v[p]);
+    v[p].vv = calloc(v[p].hdr[5] ? v[p].hdr[5] : 1, sizeof(double));
+    v[p].sbuf = calloc(v[p].hdr[6] ? v[p].hdr[6] : 1, sizeof(double));
+    for (int i = 0; i < v[p].rcount[0]; i++) v[p].vv[i] = x[v[p].idx[i]];
+  }
+  int nph = v[0].hdr[3];
+  for (int p = 0; p < np; p++) task(&v[p], 0);
+  for (int ph = 1; ph <= nph; ph++) {
+    for (int p = 0; p < np; p++) { /* gather into the send buffers */
+      const long *si = v[p].sidx + v[p].soff[ph];
+      double *sb = v[p].sbuf + v[p].soff[ph];
+      for (int i = 0; i < v[p].scount[ph]; i++) sb[i] = v[p].vv[si[i]];
+    }
+    for (int dst = 0; dst < np; dst++) /* the all-to-all exchange */
+      for (int src = 0; src < np; src++) {
+        int cnt = v[dst].rc[ph * np + src];
+        const double *from = v[src].sbuf + v[src].soff[ph] + v[src].sd[ph * np + dst];
+        double *to = v[dst].vv + v[dst].roff[ph] + v[dst].rd[ph * np + src];
+        if (cnt != v[src].sc[ph * np + dst]) {
+          fprintf(stderr, "oracle: send/recv count mismatch\n");
+          abort();
+        }
+        memcpy(to, from, sizeof(double) * cnt);
+      }
+    for (int p = 0; p < np; p++) task(&v[p], ph);
+  }
+  int n = v[0].hdr[0];
+  for (int p = 0; p < np; p++) /* collect_results */
+    for (int i = 0; i < v[p].hdr[5]; i++) vv[v[p].idx[i]] = v[p].vv[i];
+  (void)n;
+  for (int p = 0; p < np; p++) { free(v[p].vv); free(v[p].sbuf); }
+  free(v);
+}
+
+/* ------------------------------------------------------------------ */
+/* Gold: k back-to-back CSR SpMVs in CSR order, mpk2/verify_val.c:44-75 */
+/* ------------------------------------------------------------------ */
+ORC_API void orc_spmv_k(int n, const int *ptr, const int *col, const double *val,
+                        const double *x, int k, double *vv) {
+  memcpy(vv, x, sizeof(double) * n);
+  for (int l = 0; l < k; l++) {
+    const double *in = vv + (long)l * n;
+    double *out = vv + (long)(l + 1) * n;
+    for (int i = 0; i < n; i++) {
+      double s = 0.0;
+      for (int j = ptr[i]; j < ptr[i + 1]; j++) s += val[j] * in[col[j]];
+      out[i] = s;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Shared-memory path: task lists of mpk2/mpkprep.c:7-252 (prep_mpk,   */
+/* level-major l*n+i per (phase, part), then the skirt tasks), and the */
+/* valued sequential exec of mpk2/mpkvexec.c:10-55.  Returns the total */
+/* task count, or -1 when a structure check of prep_mpk fails.         */
+/* tcount[(nphase+1)*np] receives per-list sizes, tidx the indices.    */
+/* ------------------------------------------------------------------ */
+ORC_API long orc_prep_tasks(int n, const int *ptr, const int *col, int nlevel,
+                            int nphase, const int **part, const int **lev,
+                            const int *sl, int np, int *tcount, long *tidx) {
+  double *mark = malloc(sizeof(double) * (long)n * (nlevel + 1));
+  for (long i = 0; i < (long)n * (nlevel + 1); i++) mark[i] = -1.0;
+  for (int i = 0; i < n; i++) mark[i] = part[0][i] * 100.0;
+  int *zero = calloc(n, sizeof(int));
+  const int *prevl = zero;
+  int prevlmin = 0, err = 0;
+  long total = 0;
+  for (int ph = 0; ph < nphase; ph++) {
+    const int *pl = part[ph], *ll = lev[ph];
+    int lmin = ll[0], lmax = ll[0];
+    for (int i = 1; i < n; i++) {
+      if (ll[i] < lmin) lmin = ll[i];
+      if (ll[i] > lmax) lmax = ll[i];
+    }
+    if (lmax > nlevel) lmax = nlevel;
+    for (int p = 0; p < np; p++) tcount[ph * np + p] = 0;
+    for (int l = prevlmin + 1; l <= lmax; l++)
+      for (int i = 0; i < n; i++)
+        if (prevl[i] < l && l <= ll[i]) {
+          if (mark[(long)l * n + i] >= 0) err++;
+          for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+            int k = col[j];
+            double m = mark[(long)(l - 1) * n + k];
+            if (m < 0.0 || (pl[i] != pl[k] && ((int)m) % 100 == ph)) err++;
+          }
+          mark[(long)l * n + i] = pl[i] * 100.0 + ph;
+          tcount[ph * np + pl[i]]++;
+        }
+    long base = total;
+    long *start = malloc(sizeof(long) * np);
+    for (int p = 0; p < np; p++) { start[p] = base; base += tcount[ph * np + p]; }
+    for (int l = prevlmin + 1; l <= lmax; l++)
+      for (int i = 0; i < n; i++)
+        if (prevl[i] < l && l <= ll[i]) tidx[start[pl[i]]++] = (long)l * n + i;
+    free(start);
+    total = base;
+    prevl = ll;
+    prevlmin = lmin;
+  }
+  const int *pl0 = part[0];
+  for (int p = 0; p < np; p++) {
+    int cnt = 0;
+    for (int l = prevlmin + 1; l <= nlevel; l++)
+      for (int i = 0; i < n; i++) {
+        int s = sl[(long)p * n + i];
+        if (prevl[i] < l && s >= 0 && l <= nlevel - s) {
+          if (mark[(long)l * n + i] > 0.0) err++;
+          for (int j = ptr[i]; j < ptr[i + 1]; j++)
+            if (mark[(long)(l - 1) * n + col[j]] < 0.0) err++;
+          mark[(long)l * n + i] = pl0[i] * 100.0 + nphase;
+          tidx[total + cnt++] = (long)l * n + i;
+        }
+      }
+    for (int l = prevlmin + 1; l <= nlevel; l++)
+      for (int i = 0; i < n; i++) {
+        int s = sl[(long)p * n + i];
+        if (prevl[i] < l && s >= 0 && l <= nlevel - s) mark[(long)l * n + i] = -0.1;
+      }
+    tcount[nphase * np + p] = cnt;
+    total += cnt;
+  }
+  for (int i = 0; i < n; i++)
+    if (mark[(long)nlevel * n + i] < -0.5) err++;
+  free(mark); free(zero);
+  return err ? -1 : total;
+}
+
+ORC_API void orc_vexec_tasks(int n, const int *ptr, const int *col,
+                             const double *val, long ntask, const long *tidx,
+                             double *vv) {
+  for (long t = 0; t < ntask; t++) {
+    long level = tidx[t] / n;
+    long i = tidx[t] - level * n;
+    double s = 0.0;
+    for (int j = ptr[i]; j < ptr[i + 1]; j++)
+      s += val[j] * vv[(level - 1) * n + col[j]];
+    vv[level * n + i] = s;
+  }
+}

@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Regenerate the golden fixtures from the compiled reference (oracle/_ref).
+
+Runs the reference mpk2 pipeline exactly as its own harness does
+(mpk2/test_gen.sh + mpk2/driver.sh + mpk2/run_script.sh), with two
+substitutions the image forces:
+
+* partitions come from the block partitioner below (METIS/gpmetis is not in
+  the image), written in the reference `gP.part.N` text format;
+* the exec step uses oracle/_ref/ref_exec, which runs the unchanged reference
+  exec code on a seeded input vector instead of all-ones.
+
+For every case it stores, under tests/golden/<case>/:
+  inputs:   g{P}.part.{N}         partition files fed to the reference
+  outputs:  l{P}, s{nphase}       reference level / skirt text files
+            digests.json          sha256 of loop.g0, loop.val, every
+                                  rank%d.bufs, results.bin (seeded x) and
+                                  gold_result.bin (x = ones), plus sizes
+  and, for the tiny cases only (FULL below), the binary rank%d.bufs,
+  results.bin and gold_result.bin themselves.
+
+Usage: python tests/golden/make_golden.py   (needs `make -C oracle ref`)
+"""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(REPO, "oracle", "_ref")
+MPIRUN = "/opt/conda/bin/mpirun"
+SEED = 0x5EED
+
+# (type, size, npart, nlevel, nphase, grid blocks per dim)
+CASES = [
+    ("m5p", 8, 4, 4, 0, (2, 2)),
+    ("m5p", 8, 4, 4, 1, (2, 2)),
+    ("m5p", 8, 4, 4, 2, (2, 2)),
+    ("m5p", 64, 4, 4, 0, (2, 2)),
+    ("m5p", 64, 4, 4, 1, (2, 2)),
+    ("m5p", 64, 4, 4, 2, (2, 2)),
+    ("t5p", 12, 4, 4, 1, (2, 2)),
+    ("m7p", 12, 4, 4, 1, (2, 2)),
+    ("m9p", 12, 4, 4, 1, (2, 2)),
+    ("m9p", 16, 8, 6, 2, (4, 2)),
+    ("cube", 8, 8, 4, 0, (2, 2, 2)),
+    ("cube", 8, 8, 4, 1, (2, 2, 2)),
+    ("cube", 16, 8, 8, 0, (2, 2, 2)),
+    ("cube", 16, 8, 8, 1, (2, 2, 2)),
+]
+FULL = {"m5p8_4_4_0", "m5p8_4_4_1", "m5p8_4_4_2", "cube8_8_4_0", "cube8_8_4_1"}
+
+
+def splitmix_uniform(seed, n):
+    """SplitMix64 stream mapped to [-1, 1): x = 2*((z>>11)*2^-53) - 1."""
+    out = np.empty(n, dtype=np.float64)
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    m = 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & m
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        z ^= z >> 31
+        out[i] = 2.0 * ((z >> 11) * 2.0 ** -53) - 1.0
+    return out
+
+
+def block_partition(size, ndim, blocks, shift):
+    """Row-major grid (gen2 order), cartesian blocks, cyclic shift per phase."""
+    n = size ** ndim
+    idx = np.arange(n)
+    coords = []
+    for d in range(ndim):
+        coords.append((idx // size ** (ndim - 1 - d)) % size)
+    label = np.zeros(n, dtype=np.int64)
+    for d in range(ndim):
+        b = blocks[d]
+        width = size // b
+        c = (coords[d] + shift) % size
+        label = label * b + np.minimum(c // width, b - 1)
+    return label
+
+
+def sha(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()
+
+
+def run(cmd, cwd):
+    subprocess.run(cmd, cwd=cwd, check=True, stdout=subprocess.DEVNULL)
+
+
+def make_case(typ, size, npart, nlevel, nphase, blocks):
+    name = f"{typ}{size}_{npart}_{nlevel}_{nphase}"
+    ndim = 3 if typ == "cube" else 2
+    with tempfile.TemporaryDirectory() as tmp:
+        ghead = f"{typ}{size}"
+        run([os.path.join(REF, "gen"), typ, str(size), ghead], tmp)
+        run([os.path.join(REF, "gen2"), typ, str(size), ghead], tmp)
+        d = os.path.join(tmp, name)
+        os.mkdir(d)
+        shutil.copy(os.path.join(tmp, ghead + ".g0"), os.path.join(d, "g0"))
+        n = size ** ndim
+        width = size // blocks[0]
+        # driver.sh:359-380 with the partitioner standing in for gpmetis
+        parts = []
+        for ph in range(max(nphase, 1)):
+            lab = block_partition(size, ndim, blocks, (ph * width) // 2)
+            parts.append(lab)
+            with open(os.path.join(d, f"g{ph}.part.{npart}"), "w") as f:
+                f.write("".join(f"{v}\n" for v in lab))
+        for ph in range(nphase):
+            args = [os.path.join(REF, "comp"), "level", "g0", "x", f"g{ph}.part.{npart}"]
+            if ph > 0:
+                args.append(f"l{ph - 1}")
+            args.append(f"l{ph}")
+            run(args, d)
+        sk = [os.path.join(REF, "skirt"), str(nlevel), "g0", "x", f"g0.part.{npart}"]
+        if nphase > 0:
+            sk.append(f"l{nphase - 1}")
+        sk.append(f"s{nphase}")
+        run(sk, d)
+        shutil.copy(os.path.join(tmp, ghead + ".loop.g0"), os.path.join(d, "loop.g0"))
+        mpi = [MPIRUN, "-launcher", "fork", "-n", str(npart)]
+        run(mpi + [os.path.join(REF, "mpi2_mpkwrtbufs"), name], tmp)
+        x = splitmix_uniform(SEED, n)
+        x.tofile(os.path.join(tmp, "x.bin"))
+        run(mpi + [os.path.join(REF, "ref_exec"), name, "x.bin", "1"], tmp)
+        os.rename(os.path.join(d, "results.bin"), os.path.join(d, "results_seeded.bin"))
+        run(mpi + [os.path.join(REF, "mpi2_mpkexecbufs_val"), name], tmp)
+        run([os.path.join(REF, "verify_val"), name], tmp)
+        ones_equal = sha(os.path.join(d, "results.bin")) == sha(os.path.join(d, "gold_result.bin"))
+        assert ones_equal, f"{name}: reference results.bin differs from gold_result.bin"
+
+        out = os.path.join(HERE, name)
+        if os.path.isdir(out):
+            shutil.rmtree(out)
+        os.makedirs(out)
+        for ph in range(max(nphase, 1)):
+            shutil.copy(os.path.join(d, f"g{ph}.part.{npart}"), out)
+        for ph in range(nphase):
+            shutil.copy(os.path.join(d, f"l{ph}"), out)
+        shutil.copy(os.path.join(d, f"s{nphase}"), out)
+        dig = {
+            "case": name, "type": typ, "size": size, "n": n, "npart": npart,
+            "nlevel": nlevel, "nphase": nphase, "seed": SEED,
+            "loop.g0": sha(os.path.join(d, "loop.g0")),
+            "loop.val": sha(os.path.join(tmp, ghead + ".loop.val")),
+            "results_seeded.bin": sha(os.path.join(d, "results_seeded.bin")),
+            "gold_result.bin": sha(os.path.join(d, "gold_result.bin")),
+            "bufs": {},
+        }
+        for r in range(npart):
+            p = os.path.join(d, f"rank{r}.bufs")
+            dig["bufs"][str(r)] = {"sha256": sha(p), "bytes": os.path.getsize(p)}
+            if name in FULL:
+                shutil.copy(p, out)
+        if name in FULL:
+            shutil.copy(os.path.join(d, "results_seeded.bin"), out)
+            shutil.copy(os.path.join(d, "gold_result.bin"), out)
+        with open(os.path.join(out, "digests.json"), "w") as f:
+            json.dump(dig, f, indent=1, sort_keys=True)
+    print("wrote", name)
+
+
+def main():
+    if not os.path.exists(os.path.join(REF, "ref_exec")):
+        sys.exit("build the reference first: make -C oracle ref")
+    for c in CASES:
+        make_case(*c)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,209 @@
+"""ctypes access to the oracle (oracle/liboracle.so) for the test suite.
+
+The oracle is the CPU restatement of the reference path; tests use it as the
+checker only.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+GEN_TYPES = {"m5p": 0, "t5p": 1, "m7p": 2, "m9p": 3, "cube": 4}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(REPO, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run __graft_entry__.build()")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_gen2.argtypes = [C.c_int, C.c_int, P, P, P, P]
+        L.orc_gen2.restype = C.c_int
+        L.orc_uniform_pm1.argtypes = [C.c_uint64, C.c_long, P]
+        L.orc_levels.argtypes = [C.c_int, P, P, P, P, P]
+        L.orc_comp_skirt.argtypes = [C.c_int, P, P, P, P]
+        L.orc_skirt_filter.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
+        L.orc_level2wcrs.argtypes = [C.c_int, P, P, P, P]
+        L.orc_lapjv.argtypes = [P, C.c_int, P]
+        L.orc_fill_buffers.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, P, P, P, P, P]
+        L.orc_fill_buffers.restype = C.c_int
+        L.orc_free.argtypes = [P]
+        L.orc_exec_bufs.argtypes = [C.c_int, P, P, P]
+        L.orc_spmv_k.argtypes = [C.c_int, P, P, P, P, C.c_int, P]
+        L.orc_prep_tasks.argtypes = [C.c_int, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P, P]
+        L.orc_prep_tasks.restype = C.c_long
+        L.orc_vexec_tasks.argtypes = [C.c_int, P, P, P, C.c_long, P, P]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def gen2(typ, size):
+    L = lib()
+    nnz = C.c_long(0)
+    n = L.orc_gen2(GEN_TYPES[typ], size, None, None, None, C.byref(nnz))
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz.value, np.int32)
+    va = np.zeros(nnz.value, np.float64)
+    L.orc_gen2(GEN_TYPES[typ], size, ptr(rp), ptr(ci), ptr(va), C.byref(nnz))
+    return rp, ci, va
+
+
+def uniform(seed, n):
+    x = np.zeros(n, np.float64)
+    lib().orc_uniform_pm1(seed, n, ptr(x))
+    return x
+
+
+def levels(rp, ci, part, lorg=None):
+    n = len(rp) - 1
+    out = np.zeros(n, np.int32)
+    part = np.ascontiguousarray(part, np.int32)
+    lo = None if lorg is None else ptr(np.ascontiguousarray(lorg, np.int32))
+    lib().orc_levels(n, ptr(rp), ptr(ci), ptr(part), lo, ptr(out))
+    return out
+
+
+def skirt_dist(rp, ci, part):
+    n = len(rp) - 1
+    npart = int(part.max()) + 1
+    out = np.zeros(npart * n, np.int32)
+    lib().orc_comp_skirt(n, ptr(rp), ptr(ci), ptr(np.ascontiguousarray(part, np.int32)), ptr(out))
+    return out.reshape(npart, n)
+
+
+def skirt_filter(dist, nlevel, lastl=None):
+    npart, n = dist.shape
+    sl = np.zeros(npart * n, np.int32)
+    d = np.ascontiguousarray(dist, np.int32)
+    lp = None if lastl is None else ptr(np.ascontiguousarray(lastl, np.int32))
+    lib().orc_skirt_filter(n, npart, nlevel, ptr(d), lp, ptr(sl))
+    return sl.reshape(npart, n)
+
+
+def lapjv(sums):
+    s = np.ascontiguousarray(sums, np.int32)
+    npart = s.shape[0]
+    perm = np.zeros(npart, np.int32)
+    lib().orc_lapjv(ptr(s), npart, ptr(perm))
+    return perm
+
+
+def fill_buffers(rp, ci, va, nlevel, nphase, parts, levs, sl):
+    """Returns (list of rank%d.bufs byte strings, relabelled partitions)."""
+    n = len(rp) - 1
+    parts = [np.ascontiguousarray(p, np.int32).copy() for p in parts]
+    levs = [np.ascontiguousarray(l, np.int32) for l in levs]
+    npart = int(parts[0].max()) + 1
+    pp = (C.c_void_p * len(parts))(*[p.ctypes.data for p in parts])
+    lp = (C.c_void_p * max(len(levs), 1))(*[l.ctypes.data for l in levs])
+    blobs = (C.c_void_p * npart)()
+    lens = np.zeros(npart, np.int64)
+    sl = np.ascontiguousarray(sl, np.int32)
+    lib().orc_fill_buffers(n, ptr(rp), ptr(ci), ptr(va), nlevel, nphase, pp, lp,
+                           ptr(sl), blobs, ptr(lens))
+    out = []
+    for r in range(npart):
+        out.append(C.string_at(blobs[r], int(lens[r])))
+        lib().orc_free(blobs[r])
+    return out, parts
+
+
+def exec_bufs(blobs, x, nlevel):
+    n = len(x)
+    arr = [np.frombuffer(b, np.uint8).copy() for b in blobs]
+    bp = (C.c_void_p * len(arr))(*[a.ctypes.data for a in arr])
+    vv = np.zeros(n * (nlevel + 1), np.float64)
+    lib().orc_exec_bufs(len(arr), bp, ptr(np.ascontiguousarray(x)), ptr(vv))
+    return vv
+
+
+def spmv_k(rp, ci, va, x, k):
+    n = len(rp) - 1
+    vv = np.zeros(n * (k + 1), np.float64)
+    lib().orc_spmv_k(n, ptr(rp), ptr(ci), ptr(va), ptr(np.ascontiguousarray(x)), k, ptr(vv))
+    return vv
+
+
+def prep_tasks(rp, ci, nlevel, nphase, parts, levs, sl):
+    n = len(rp) - 1
+    npart = sl.shape[0]
+    parts = [np.ascontiguousarray(p, np.int32) for p in parts]
+    levs = [np.ascontiguousarray(l, np.int32) for l in levs]
+    pp = (C.c_void_p * len(parts))(*[p.ctypes.data for p in parts])
+    lp = (C.c_void_p * max(len(levs), 1))(*[l.ctypes.data for l in levs])
+    cap = n * nlevel + int((sl >= 0).sum()) * (nlevel + 1)
+    tcount = np.zeros((nphase + 1) * npart, np.int32)
+    tidx = np.zeros(cap, np.int64)
+    tot = lib().orc_prep_tasks(n, ptr(rp), ptr(ci), nlevel, nphase, pp, lp,
+                               ptr(np.ascontiguousarray(sl, np.int32)), npart,
+                               ptr(tcount), ptr(tidx))
+    return tot, tcount, tidx[:max(tot, 0)]
+
+
+def vexec_tasks(rp, ci, va, tidx, x, nlevel):
+    n = len(rp) - 1
+    vv = np.zeros(n * (nlevel + 1))
+    vv[:n] = x
+    lib().orc_vexec_tasks(n, ptr(rp), ptr(ci), ptr(va), len(tidx), ptr(tidx), ptr(vv))
+    return vv
+
+
+# ---- golden fixture helpers -------------------------------------------------
+
+def cases():
+    return sorted(d for d in os.listdir(GOLDEN)
+                  if os.path.isfile(os.path.join(GOLDEN, d, "digests.json")))
+
+
+def digests(case):
+    with open(os.path.join(GOLDEN, case, "digests.json")) as f:
+        return json.load(f)
+
+
+def read_ints(path):
+    with open(path) as f:
+        return np.array([int(t) for t in f.read().split()], np.int32)
+
+
+def read_skirt_text(path, n, nlevel):
+    """read_skirt semantics (mpk2/lib.c:656-707): sl = nlevel - m, -1 if absent."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    head = lines[0].split()
+    npart = int(head[3])
+    sl = -np.ones((npart, n), np.int32)
+    p = -1
+    for ln in lines[1:]:
+        if ln.startswith("partition"):
+            p = int(ln.split()[1])
+            continue
+        j, m = ln.split()
+        sl[p, int(j)] = nlevel - int(m)
+    return sl
+
+
+def sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+def case_inputs(case):
+    """gen2 matrix, partitions, reference level/skirt files of a golden case."""
+    d = digests(case)
+    rp, ci, va = gen2(d["type"], d["size"])
+    base = os.path.join(GOLDEN, case)
+    npart, nphase = d["npart"], d["nphase"]
+    parts = [read_ints(os.path.join(base, f"g{ph}.part.{npart}")) for ph in range(max(nphase, 1))]
+    levs = [read_ints(os.path.join(base, f"l{ph}")) for ph in range(nphase)]
+    return d, rp, ci, va, parts, levs

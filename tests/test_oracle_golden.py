@@ -1,0 +1,192 @@
+"""Pins the oracle (oracle/oracle.c) to the reference's own outputs.
+
+Every golden case was produced by the compiled reference mpk2 programs
+(tests/golden/make_golden.py): gen2 matrices, `comp level` l-files, `skirt`
+s-files, mpi2_mpkwrtbufs rank%d.bufs, the unchanged reference exec on a seeded
+x (results_seeded.bin) and verify_val on x = ones (gold_result.bin).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import tests.oracle_api as O
+
+CASES = O.cases()
+
+
+def gen2_text(rp, ci, va):
+    """loop.g0 / loop.val text exactly as mpk2/gen2.c prints them."""
+    n = len(rp) - 1
+    nloops = sum(1 for i in range(n) for j in range(rp[i], rp[i + 1]) if ci[j] == i)
+    head = f"{n} {(len(ci) + nloops) // 2}\n"
+    rows = ["".join(f"{c + 1} " for c in ci[rp[i]:rp[i + 1]]) + "\n" for i in range(n)]
+    vals = "".join("%f " % v for v in va)
+    return (head + "".join(rows)).encode(), vals.encode()
+
+
+def skirt_text(sl_dist_filtered, nlevel):
+    npart, n = sl_dist_filtered.shape
+    out = [f"skirt {n} nodes, {npart} parts\n"]
+    for p in range(npart):
+        out.append(f"partition {p}\n")
+        for j in np.nonzero(sl_dist_filtered[p] >= 0)[0]:
+            out.append(f"{j} {nlevel - sl_dist_filtered[p, j]}\n")
+    return "".join(out).encode()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_gen2_matches_reference(case):
+    d = O.digests(case)
+    rp, ci, va = O.gen2(d["type"], d["size"])
+    g, v = gen2_text(rp, ci, va)
+    assert O.sha(g) == d["loop.g0"]
+    assert O.sha(v) == d["loop.val"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_levels_and_skirt_match_reference(case, golden_dir):
+    d, rp, ci, va, parts, levs = O.case_inputs(case)
+    prev = None
+    for ph in range(d["nphase"]):
+        lv = O.levels(rp, ci, parts[ph], prev)
+        np.testing.assert_array_equal(lv, levs[ph])
+        prev = lv
+    dist = O.skirt_dist(rp, ci, parts[0])
+    sl = O.skirt_filter(dist, d["nlevel"], levs[-1] if d["nphase"] else None)
+    with open(f"{golden_dir}/{case}/s{d['nphase']}", "rb") as f:
+        assert skirt_text(sl, d["nlevel"]) == f.read()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fill_buffers_byte_equal_reference(case):
+    d, rp, ci, va, parts, levs = O.case_inputs(case)
+    dist = O.skirt_dist(rp, ci, parts[0])
+    sl = O.skirt_filter(dist, d["nlevel"], levs[-1] if d["nphase"] else None)
+    blobs, _ = O.fill_buffers(rp, ci, va, d["nlevel"], d["nphase"], parts, levs, sl)
+    for r, b in enumerate(blobs):
+        ref = d["bufs"][str(r)]
+        assert len(b) == ref["bytes"], f"rank {r} size"
+        assert O.sha(b) == ref["sha256"], f"rank {r} content"
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_exec_bitwise_equal_reference(case):
+    d, rp, ci, va, parts, levs = O.case_inputs(case)
+    dist = O.skirt_dist(rp, ci, parts[0])
+    sl = O.skirt_filter(dist, d["nlevel"], levs[-1] if d["nphase"] else None)
+    blobs, _ = O.fill_buffers(rp, ci, va, d["nlevel"], d["nphase"], parts, levs, sl)
+    x = O.uniform(d["seed"], d["n"])
+    vv = O.exec_bufs(blobs, x, d["nlevel"])
+    assert O.sha(vv.tobytes()) == d["results_seeded.bin"]
+    gold = O.spmv_k(rp, ci, va, x, d["nlevel"])
+    assert O.sha(gold.tobytes()) == d["results_seeded.bin"]
+    ones = O.spmv_k(rp, ci, va, np.ones(d["n"]), d["nlevel"])
+    assert O.sha(ones.tobytes()) == d["gold_result.bin"]
+
+
+def test_uniform_matches_python_restatement():
+    from tests.golden.make_golden import splitmix_uniform
+    np.testing.assert_array_equal(O.uniform(0x5EED, 1000), splitmix_uniform(0x5EED, 1000))
+    x = O.uniform(7, 100000)
+    assert x.min() >= -1.0 and x.max() < 1.0 and abs(x.mean()) < 0.01
+
+
+def test_levels_closed_form_slabs():
+    # 1D chain split in two slabs: level = distance to the other slab - 1.
+    n = 20
+    rp = np.arange(0, 3 * n - 1, 1, dtype=np.int32)
+    cols = []
+    rps = [0]
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                cols.append(j)
+        rps.append(len(cols))
+    rp = np.array(rps, np.int32)
+    ci = np.array(cols, np.int32)
+    part = (np.arange(n) >= 8).astype(np.int32)
+    lv = O.levels(rp, ci, part)
+    expect = np.where(np.arange(n) < 8, 7 - np.arange(n), np.arange(n) - 8)
+    np.testing.assert_array_equal(lv, expect)
+    dist = O.skirt_dist(rp, ci, part)
+    np.testing.assert_array_equal(dist[0], np.maximum(np.arange(n) - 7, 0))
+
+
+def test_skirt_brute_force_bfs():
+    rp, ci, va = O.gen2("m9p", 7)
+    n = len(rp) - 1
+    rng = np.random.default_rng(3)
+    part = rng.integers(0, 3, n).astype(np.int32)
+    part[:3] = [0, 1, 2]
+    dist = O.skirt_dist(rp, ci, part)
+    for p in range(3):
+        d = np.full(n, n + 1)
+        frontier = list(np.nonzero(part == p)[0])
+        d[frontier] = 0
+        while frontier:
+            nxt = []
+            for i in frontier:
+                for j in ci[rp[i]:rp[i + 1]]:
+                    if d[j] > d[i] + 1:
+                        d[j] = d[i] + 1
+                        nxt.append(j)
+            frontier = nxt
+        np.testing.assert_array_equal(dist[p], d)
+
+
+def test_lapjv_optimal_like_reference_gtest():
+    # lab/lapjv_test.cc:149-154: random 5x5 integer matrices, the assignment
+    # maximises the trace (compared with brute force over all permutations).
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        s = rng.integers(0, 20, (5, 5)).astype(np.int32)
+        perm = O.lapjv(s)
+        assert sorted(perm) == list(range(5))
+        got = sum(s[perm[j], j] for j in range(5))
+        best = max(sum(s[p[j], j] for j in range(5)) for p in itertools.permutations(range(5)))
+        assert got == best
+
+
+def test_lapjv_fixture_csum_xenon2():
+    # lab/csum_xenon2_4_10_3_{1,2}.txt are 4x4 communication-sum matrices.
+    for k in (1, 2):
+        path = f"{O.GOLDEN}/lapjv/csum_xenon2_4_10_3_{k}.txt"
+        raw = O.read_ints(path)  # first int is the size (lab/lapjv_test.cc:69-79)
+        s = raw[1:].reshape(raw[0], raw[0])
+        perm = O.lapjv(s)
+        best = max(sum(s[p[j], j] for j in range(4)) for p in itertools.permutations(range(4)))
+        assert sum(s[perm[j], j] for j in range(4)) == best
+
+
+def test_task_lists_reproduce_gold():
+    for case in ["m5p8_4_4_0", "m5p8_4_4_2", "cube8_8_4_1", "m9p16_8_6_2"]:
+        d, rp, ci, va, parts, levs = O.case_inputs(case)
+        dist = O.skirt_dist(rp, ci, parts[0])
+        sl = O.skirt_filter(dist, d["nlevel"], levs[-1] if d["nphase"] else None)
+        tot, tcount, tidx = O.prep_tasks(rp, ci, d["nlevel"], d["nphase"], parts, levs, sl)
+        assert tot >= d["n"] * d["nlevel"]
+        x = O.uniform(d["seed"], d["n"])
+        vv = O.vexec_tasks(rp, ci, va, tidx, x, d["nlevel"])
+        assert O.sha(vv.tobytes()) == d["results_seeded.bin"]
+
+
+def test_oracle_detects_planted_mistakes():
+    """A wrong level file or a transposed matrix must change the outputs."""
+    d, rp, ci, va, parts, levs = O.case_inputs("m5p8_4_4_1")
+    x = O.uniform(d["seed"], d["n"])
+    gold = O.spmv_k(rp, ci, va, x, d["nlevel"])
+    # plant: drop the last entry of row 5 -> results must differ
+    va2 = va.copy()
+    va2[rp[6] - 1] = 0.0
+    assert not np.array_equal(O.spmv_k(rp, ci, va2, x, d["nlevel"]), gold)
+    # plant: skirt computed with one partition label swapped -> bufs differ
+    dist = O.skirt_dist(rp, ci, parts[0])
+    sl = O.skirt_filter(dist, d["nlevel"], levs[-1])
+    blobs, _ = O.fill_buffers(rp, ci, va, d["nlevel"], d["nphase"], parts, levs, sl)
+    p2 = [p.copy() for p in parts]
+    p2[0][0], p2[0][63] = p2[0][63], p2[0][0]
+    dist2 = O.skirt_dist(rp, ci, p2[0])
+    sl2 = O.skirt_filter(dist2, d["nlevel"], levs[-1])
+    blobs2, _ = O.fill_buffers(rp, ci, va, d["nlevel"], d["nphase"], p2, levs, sl2)
+    assert O.sha(blobs[0]) != O.sha(blobs2[0])
