@@ -1,0 +1,264 @@
+/*
+ * mpkx.h -- C ABI of the MI355X matrix powers kernel (MPK) library
+ * libmpkx.so.  Computes the Krylov block [x, Ax, ..., A^k x] under the
+ * partitioned, phased schedule of vatai/mpk's mpk2 code, with the exec path
+ * running as hand-written HIP kernels on one gfx950 GPU.
+ *
+ * Conventions
+ *   - Every entry point that can fail returns 0 (MPKX_OK) or a negative
+ *     MPKX_E* code and never exits the process; mpkx_last_error() returns a
+ *     message for the calling thread's last failure.  (The reference uses
+ *     assert()/exit(1) instead, mpk2/buffers.c:639-642, mpk2/lib.c:76-84.)
+ *   - Host arrays stay owned by the caller unless a function says it
+ *     allocates (then release with the named *_free / del function).
+ *   - Pointers documented as "device" are HIP device pointers (hipMalloc'ed
+ *     or from a framework allocator); `stream` is a hipStream_t passed as
+ *     void* (NULL = the default stream).
+ *   - A plan is used by one host thread at a time.
+ *   - Level-major Krylov block: vv[l*n + i] = (A^l x)_i, l = 0..nlevel,
+ *     the layout of the reference results.bin (mpk2/mpi2_mpkexecbufs_val.c:70-74).
+ */
+#ifndef MPKX_H
+#define MPKX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+enum {
+  MPKX_OK = 0,
+  MPKX_EINVAL = -1,      /* bad argument or inconsistent input           */
+  MPKX_ENOMEM = -2,      /* host or device allocation failed             */
+  MPKX_EIO = -3,         /* file could not be opened / parsed            */
+  MPKX_EHIP = -4,        /* a HIP runtime call failed                    */
+  MPKX_ESTRUCT = -5,     /* schedule structure error (prep_mpk checks)   */
+  MPKX_EUNSUPPORTED = -6 /* input outside what a kernel variant handles  */
+};
+
+/* Message of the last failure on this thread ("" if none). */
+const char *mpkx_last_error(void);
+/* Library version string. */
+const char *mpkx_version(void);
+/* Releases memory the library allocated for the caller. */
+void mpkx_free(void *p);
+
+/* ------------------------------------------------------------------ */
+/* Per-partition exec program.  Same field order, types and meaning as */
+/* the reference `buffers_t` (mpk2/buffers.h:18-100), so a reference   */
+/* caller may pass its buffers_t* where mpkx_buffers* is expected.     */
+/* vv_buf of one partition is laid out per phase as [rbuf | mbuf].     */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_buffers {
+  int n, npart, nlevel, nphase, rank;
+  int buf_count, buf_scount, mptr_count, mcol_count;
+  int *recvcounts, *sendcounts, *rdispls, *sdispls; /* npart*(nphase+1) */
+  int *rcount, *mcount, *scount;                    /* nphase+1 */
+  int *rbuf_offsets, *mbuf_offsets, *sbuf_offsets;  /* nphase+1 */
+  int *mptr_offsets, *mcol_offsets;                 /* nphase+1 */
+  long *idx_buf;  /* buf_count: level*n + i of every vv_buf slot          */
+  long *idx_sbuf; /* buf_scount: vv_buf slot copied into each send slot   */
+  long *mptr_buf; /* mptr_count                                           */
+  long *mcol_buf; /* mcol_count: vv_buf slot of every matrix entry        */
+  double *vv_buf, *vv_sbuf, *mval_buf;
+} mpkx_buffers;
+
+/* read_buffers / write_buffers / alloc_val_bufs / del_bufs of
+ * mpk2/buffers.c:588-697, :533-571.  The file is DIR/rank%d.bufs in the
+ * reference binary layout; mval_buf is optional (absent -> NULL).
+ * mpkx_read_buffers returns NULL on failure (see mpkx_last_error). */
+mpkx_buffers *mpkx_read_buffers(const char *dir, int rank);
+int mpkx_write_buffers(const mpkx_buffers *bufs, const char *dir);
+int mpkx_alloc_val_bufs(mpkx_buffers *bufs);
+void mpkx_del_bufs(mpkx_buffers *bufs);
+
+/* ------------------------------------------------------------------ */
+/* Inputs and formats (host)                                           */
+/* ------------------------------------------------------------------ */
+/* read_crs (mpk2/lib.c:69-129): METIS-style text graph, 1-based columns.
+ * Allocates *ptr (n+1) and *col (nnz); free both with mpkx_free. */
+int mpkx_read_crs(const char *path, int *n, int **ptr, int **col);
+/* Writes the same format; header "n ne" with ne = (nnz + #self-loops)/2
+ * (the header mpk2/gen2.c prints). */
+int mpkx_write_crs(const char *path, int n, const int *ptr, const int *col);
+/* alloc_read_val (mpk2/comm_data.c:160-187): nnz "%lf " values. */
+int mpkx_read_vals(const char *path, long nnz, double *val);
+int mpkx_write_vals(const char *path, long nnz, const double *val);
+/* Matrix Market coordinate file -> CSR (pdmpk/csr.cc:84-121 semantics:
+ * symmetric entries mirrored, pattern matrices get 1/deg values, rows in
+ * file order).  Allocates *ptr, *col, *val (mpkx_free). */
+int mpkx_read_mtx(const char *path, int *n, int **ptr, int **col,
+                  double **val);
+/* read_part / read_level (mpk2/lib.c:273-284, :414-434): one int per line.
+ * *npart = max label + 1 (may be NULL). */
+int mpkx_read_ints(const char *path, int n, int *out, int *npart);
+int mpkx_write_ints(const char *path, int n, const int *v);
+
+/* Stencil generators with the row and column order of mpk2/gen2.c:59-268.
+ * type: 0=m5p 1=t5p 2=m7p 3=m9p 4=cube.  Values are gen2's (mid -4/-6/-8,
+ * side +1) multiplied by `scale` (scale = -1 gives the diag +4/+6,
+ * off-diagonal -1 convention).  Call with NULL arrays to get n and nnz. */
+int mpkx_gen2(int type, int size, double scale, int *n, long *nnz, int *ptr,
+              int *col, double *val);
+/* SplitMix64 (seed) -> x_i = 2*((z>>11)*2^-53) - 1, i < n. */
+void mpkx_uniform_pm1(uint64_t seed, long n, double *x);
+
+/* ------------------------------------------------------------------ */
+/* Partitions, levels and skirt (host).  The reference calls METIS     */
+/* (mpk2/driver.c:63-66); these partitioners stand in for it and write */
+/* the same gP.part.N labels.                                          */
+/* ------------------------------------------------------------------ */
+/* Cartesian blocks of a row-major grid dims[0..ndim) (slowest first),
+ * blocks[d] blocks along dimension d, coordinates cyclically shifted by
+ * shift[d] before blocking.  Labels in [0, prod(blocks)). */
+int mpkx_partition_grid(int ndim, const int *dims, const int *blocks,
+                        const int *shift, int *part);
+/* Graph partitioner for general matrices: breadth-first region growing
+ * from pseudo-peripheral seeds into npart connected, balanced parts. */
+int mpkx_partition_graph(int n, const int *ptr, const int *col, int npart,
+                         int *part);
+
+/* comp_level (mpk2/lib.c:330-371) when lorg == NULL, else update_level
+ * (mpk2/lib.c:373-412); bit-exact, including the "big" values of rows
+ * whose partition never touches another one.  Self loops are ignored as
+ * the reference's min-plus iteration ignores them. */
+int mpkx_levels(int n, const int *ptr, const int *col, const int *part,
+                const int *lorg, int *level);
+/* Edge weights of mpk2/lib.c:504-537 (input of the next partitioning). */
+int mpkx_level2wcrs(int n, const int *ptr, const int *col, const int *level,
+                    int *we);
+
+/* Skirt: comp_skirt (mpk2/lib.c:584-615) filtered exactly as write_skirt
+ * + read_skirt (mpk2/lib.c:617-707) keep it: row j belongs to partition
+ * p's skirt with distance d = dist(j, p) iff d < nlevel (lastl == NULL,
+ * nphase 0) or lastl[j] < nlevel - d.  Sparse result: for partition p the
+ * rows srow[sptr[p] .. sptr[p+1]) in ascending order and their distances
+ * sdist[...].  Computed by breadth-first search bounded at depth nlevel-1.
+ * Allocates *sptr (npart+1), *srow, *sdist (mpkx_free). */
+int mpkx_comp_skirt(int n, const int *ptr, const int *col, const int *part,
+                    int npart, int nlevel, const int *lastl, long **sptr,
+                    int **srow, int **sdist);
+/* Text form of the s-file (write_skirt's layout, mpk2/lib.c:617-654). */
+int mpkx_write_skirt(const char *path, int n, int npart, int nlevel,
+                     const long *sptr, const int *srow, const int *sdist);
+/* Reads an s-file back into the sparse form (read_skirt semantics). */
+int mpkx_read_skirt(const char *path, int n, int npart, int nlevel,
+                    long **sptr, int **srow, int **sdist);
+
+/* Jonker-Volgenant assignment maximising trace(sums) as mpk2/lapjv.c:366
+ * (same tie-breaking); perm[new label] = old label. */
+int mpkx_lapjv(const int *sums, int npart, int *perm);
+
+/* ------------------------------------------------------------------ */
+/* Buffer prep: fill_buffers + alloc_fill_mval for every rank at once  */
+/* (mpk2/buffers.c:573-586, mpk2/mpi2_mpkwrtbufs.c:12-30), byte-equal  */
+/* to the reference rank%d.bufs.  parts[ph] (ph < max(nphase,1)) are   */
+/* relabelled in place by the LAPJV step of phases >= 1 exactly as the */
+/* reference relabels its copy (mpk2/reduce_comm.c:67-82).  levels[ph] */
+/* for ph < nphase.  Skirt in the sparse form of mpkx_comp_skirt.      */
+/* val may be NULL (no mval_buf).  On success *out receives an array of */
+/* npart buffers (release with mpkx_del_bufs_array).                   */
+/* ------------------------------------------------------------------ */
+int mpkx_fill_buffers(int n, const int *ptr, const int *col,
+                      const double *val, int nlevel, int nphase, int **parts,
+                      const int *const *levels, const long *sptr,
+                      const int *srow, const int *sdist, int npart,
+                      mpkx_buffers ***out);
+void mpkx_del_bufs_array(mpkx_buffers **bufs, int npart);
+
+/* Shared-memory schedule of mpk2/mpkprep.c:7-252 (prep_mpk): task lists
+ * of l*n+i per (phase, partition) then the skirt tasks, after the same
+ * structure checks (MPKX_ESTRUCT when one fails).  tcount receives
+ * (nphase+1)*npart list sizes; *tidx (allocated, mpkx_free) the indices;
+ * *ntask the total. */
+int mpkx_prep_mpk(int n, const int *ptr, const int *col, int nlevel,
+                  int nphase, const int *const *parts,
+                  const int *const *levels, const long *sptr,
+                  const int *srow, const int *sdist, int npart, int *tcount,
+                  long **tidx, long *ntask);
+
+/* ------------------------------------------------------------------ */
+/* Exec of reference buffers on the GPU (drop-in for P MPI ranks each  */
+/* running mpk_exec_bufs_val, mpk2/mpi2_mpkexecbufs_val.c:13-56).      */
+/* do_task runs as level-synchronous HIP kernels over all partitions;  */
+/* the MPI_Alltoallv of do_comm becomes a ghost gather inside one flat */
+/* HBM array holding every partition's vv_buf.  Results are bitwise    */
+/* equal to the reference (same summation order, no fused multiply-add).*/
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_plan mpkx_plan;
+
+/* Uploads the programs of bufs[0..npart) (rank p at index p; every
+ * bufs[p]->mval_buf must be set). */
+int mpkx_plan_create(mpkx_plan **plan, mpkx_buffers *const *bufs, int npart);
+void mpkx_plan_destroy(mpkx_plan *plan);
+/* Host drop-in: takes each bufs[p]->vv_buf[0 .. rcount[0]) as the input
+ * (as the reference main loads it), runs every phase, and writes every
+ * bufs[p]->vv_buf back (vv_buf must be allocated, see mpkx_alloc_val_bufs). */
+int mpkx_exec_bufs(mpkx_plan *plan, mpkx_buffers *const *bufs);
+/* Device form: x (device, n doubles) -> vv (device, n*(nlevel+1)),
+ * loading rbuf[0] = x[idx_buf] and collecting like collect_results
+ * (mpk2/mpi2_mpkexec.c:122-146). */
+int mpkx_exec(mpkx_plan *plan, const double *x, double *vv, void *stream);
+/* Statistics: rows computed, matrix entries used, ghost values moved,
+ * kernel launches per exec. */
+int mpkx_plan_stats(const mpkx_plan *plan, long *rows, long *nnz,
+                    long *ghosts, int *launches);
+
+/* ------------------------------------------------------------------ */
+/* Plain k x SpMV baseline (mpk2/verify_val.c:44-75): vv[(l+1)n ..] =   */
+/* A vv[l n ..] for l < k, CSR order, bitwise equal to verify_val.      */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_csr mpkx_csr;
+int mpkx_csr_create(mpkx_csr **m, int n, const int *ptr, const int *col,
+                    const double *val);
+void mpkx_csr_destroy(mpkx_csr *m);
+/* vv (device) already holds x in vv[0..n).  use_graph != 0 records the k
+ * launches once into a hipGraph (per vv/stream) and replays it. */
+int mpkx_spmv_k(mpkx_csr *m, double *vv, int k, void *stream, int use_graph);
+
+/* ------------------------------------------------------------------ */
+/* Fused MPK: "skirt rounds".  Every workgroup owns one partition       */
+/* (tile), stages the tile plus its skirt (rows within graph distance s)*/
+/* in LDS and advances s powers per launch, redundantly recomputing the */
+/* shrinking skirt instead of exchanging halos (the nphase = 0 schedule */
+/* of the reference applied every s levels).  Between launches the halo */
+/* is gathered from the Krylov block in HBM.  FP64 results are bitwise */
+/* equal to k x SpMV in CSR order.                                      */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_fused mpkx_fused;
+enum {
+  MPKX_FUSED_AUTO = 0,     /* pick the value codec automatically          */
+  MPKX_FUSED_EXPLICIT = 1, /* force explicit FP64 values                  */
+};
+/* part[n] labels in [0, npart); s = levels per launch (1..8). */
+int mpkx_fused_create(mpkx_fused **f, int n, const int *ptr, const int *col,
+                      const double *val, const int *part, int npart,
+                      int nlevel, int s, int flags);
+void mpkx_fused_destroy(mpkx_fused *f);
+/* x, vv device; vv[0..n) receives x.  use_graph as in mpkx_spmv_k. */
+int mpkx_fused_exec(mpkx_fused *f, const double *x, double *vv, void *stream,
+                    int use_graph);
+/* Launches one round only: levels L0+1 .. L0+w with L0 = round * s
+ * (vv[L0*n ..] must already hold level L0).  For per-kernel timing. */
+int mpkx_fused_round(mpkx_fused *f, double *vv, int round, void *stream);
+/* Plan statistics: max slots per tile, total slots, computed row-levels
+ * per exec, matrix bytes, codec (0 explicit FP64, 1 dictionary),
+ * launches per exec. */
+int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots,
+                     long *row_levels, long *matrix_bytes, int *codec,
+                     int *launches);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPKX_H */

@@ -1,0 +1,507 @@
+"""mpk_amd -- MI355X matrix powers kernel (MPK) library, Python host side.
+
+Thin ctypes mirror of libmpkx.so (C ABI in include/mpkx.h).  The names follow
+the reference mpk2 entry points they stand for (comp_level, update_level,
+comp_skirt, fill_buffers, read_buffers, write_buffers, prep_mpk, ...).  The
+exec path is the HIP code inside libmpkx.so; there is no CPU fallback: every
+GPU entry point raises when the library or a GPU is missing.
+
+Device memory is passed as raw HIP device pointers; torch tensors are
+accepted (their data_ptr() is used) so callers can let torch allocate.
+"""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmpkx.so")
+GEN_TYPES = {"m5p": 0, "t5p": 1, "m7p": 2, "m9p": 3, "cube": 4}
+
+
+class MpkxError(RuntimeError):
+    """A libmpkx entry point returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"mpkx error {code}: {msg}")
+        self.code = code
+
+
+class Buffers(C.Structure):
+    """mpkx_buffers: field order and types of the reference buffers_t."""
+    _fields_ = [(f, C.c_int) for f in (
+        "n", "npart", "nlevel", "nphase", "rank",
+        "buf_count", "buf_scount", "mptr_count", "mcol_count")] + \
+        [(f, C.POINTER(C.c_int)) for f in (
+            "recvcounts", "sendcounts", "rdispls", "sdispls", "rcount", "mcount",
+            "scount", "rbuf_offsets", "mbuf_offsets", "sbuf_offsets",
+            "mptr_offsets", "mcol_offsets")] + \
+        [(f, C.POINTER(C.c_long)) for f in ("idx_buf", "idx_sbuf", "mptr_buf", "mcol_buf")] + \
+        [(f, C.POINTER(C.c_double)) for f in ("vv_buf", "vv_sbuf", "mval_buf")]
+
+
+BufPtr = C.POINTER(Buffers)
+_lib = None
+P = C.c_void_p
+
+
+def lib():
+    """Loads libmpkx.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIBPATH)
+    sig = {
+        "mpkx_last_error": ([], C.c_char_p),
+        "mpkx_version": ([], C.c_char_p),
+        "mpkx_free": ([P], None),
+        "mpkx_read_buffers": ([C.c_char_p, C.c_int], BufPtr),
+        "mpkx_write_buffers": ([BufPtr, C.c_char_p], C.c_int),
+        "mpkx_alloc_val_bufs": ([BufPtr], C.c_int),
+        "mpkx_del_bufs": ([BufPtr], None),
+        "mpkx_read_crs": ([C.c_char_p, P, P, P], C.c_int),
+        "mpkx_write_crs": ([C.c_char_p, C.c_int, P, P], C.c_int),
+        "mpkx_read_vals": ([C.c_char_p, C.c_long, P], C.c_int),
+        "mpkx_write_vals": ([C.c_char_p, C.c_long, P], C.c_int),
+        "mpkx_read_mtx": ([C.c_char_p, P, P, P, P], C.c_int),
+        "mpkx_read_ints": ([C.c_char_p, C.c_int, P, P], C.c_int),
+        "mpkx_write_ints": ([C.c_char_p, C.c_int, P], C.c_int),
+        "mpkx_gen2": ([C.c_int, C.c_int, C.c_double, P, P, P, P, P], C.c_int),
+        "mpkx_uniform_pm1": ([C.c_uint64, C.c_long, P], None),
+        "mpkx_partition_grid": ([C.c_int, P, P, P, P], C.c_int),
+        "mpkx_partition_graph": ([C.c_int, P, P, C.c_int, P], C.c_int),
+        "mpkx_levels": ([C.c_int, P, P, P, P, P], C.c_int),
+        "mpkx_level2wcrs": ([C.c_int, P, P, P, P], C.c_int),
+        "mpkx_comp_skirt": ([C.c_int, P, P, P, C.c_int, C.c_int, P, P, P, P], C.c_int),
+        "mpkx_write_skirt": ([C.c_char_p, C.c_int, C.c_int, C.c_int, P, P, P], C.c_int),
+        "mpkx_read_skirt": ([C.c_char_p, C.c_int, C.c_int, C.c_int, P, P, P], C.c_int),
+        "mpkx_lapjv": ([P, C.c_int, P], C.c_int),
+        "mpkx_fill_buffers": ([C.c_int, P, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P], C.c_int),
+        "mpkx_del_bufs_array": ([P, C.c_int], None),
+        "mpkx_prep_mpk": ([C.c_int, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P, P, P], C.c_int),
+        "mpkx_plan_create": ([P, P, C.c_int], C.c_int),
+        "mpkx_plan_destroy": ([P], None),
+        "mpkx_exec_bufs": ([P, P], C.c_int),
+        "mpkx_exec": ([P, P, P, P], C.c_int),
+        "mpkx_plan_stats": ([P, P, P, P, P], C.c_int),
+        "mpkx_csr_create": ([P, C.c_int, P, P, P], C.c_int),
+        "mpkx_csr_destroy": ([P], None),
+        "mpkx_spmv_k": ([P, P, C.c_int, P, C.c_int], C.c_int),
+        "mpkx_fused_create": ([P, C.c_int, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "mpkx_fused_destroy": ([P], None),
+        "mpkx_fused_exec": ([P, P, P, P, C.c_int], C.c_int),
+        "mpkx_fused_stats": ([P, P, P, P, P, P, P], C.c_int),
+        "mpkx_fused_round": ([P, P, C.c_int, P], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise MpkxError(rc, lib().mpkx_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(P)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _take(ptr_value, count, dtype):
+    """Copies a library-allocated array into numpy and frees it."""
+    if count == 0:
+        out = np.zeros(0, dtype)
+    else:
+        ct = {np.int32: C.c_int, np.int64: C.c_long, np.float64: C.c_double}[dtype]
+        out = np.ctypeslib.as_array(C.cast(ptr_value, C.POINTER(ct)), (count,)).copy()
+    lib().mpkx_free(ptr_value)
+    return out
+
+
+def _dev(t):
+    """Device pointer of a torch tensor or an int address."""
+    if hasattr(t, "data_ptr"):
+        return C.c_void_p(t.data_ptr())
+    return C.c_void_p(int(t))
+
+
+def _stream(stream):
+    if stream is None:
+        try:
+            import torch
+            return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except Exception:
+            return C.c_void_p(0)
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+# ---- inputs ----------------------------------------------------------------
+
+def gen2(typ, size, scale=1.0):
+    """Stencil matrix in mpk2/gen2.c order; scale=-1 gives diag +4/+6/+8, off -1."""
+    L = lib()
+    n = C.c_int()
+    nnz = C.c_long()
+    _check(L.mpkx_gen2(GEN_TYPES[typ], size, scale, C.byref(n), C.byref(nnz), None, None, None))
+    rp = np.zeros(n.value + 1, np.int32)
+    ci = np.zeros(nnz.value, np.int32)
+    va = np.zeros(nnz.value, np.float64)
+    _check(L.mpkx_gen2(GEN_TYPES[typ], size, scale, C.byref(n), C.byref(nnz), _p(rp), _p(ci), _p(va)))
+    return rp, ci, va
+
+
+def uniform_pm1(seed, n):
+    x = np.zeros(n, np.float64)
+    lib().mpkx_uniform_pm1(seed, n, _p(x))
+    return x
+
+
+def read_crs(path):
+    n = C.c_int()
+    rp = C.c_void_p()
+    ci = C.c_void_p()
+    _check(lib().mpkx_read_crs(path.encode(), C.byref(n), C.byref(rp), C.byref(ci)))
+    ptr = _take(rp, n.value + 1, np.int32)
+    return ptr, _take(ci, int(ptr[-1]), np.int32)
+
+
+def write_crs(path, ptr, col):
+    _check(lib().mpkx_write_crs(path.encode(), len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col))))
+
+
+def read_vals(path, nnz):
+    v = np.zeros(nnz, np.float64)
+    _check(lib().mpkx_read_vals(path.encode(), nnz, _p(v)))
+    return v
+
+
+def write_vals(path, val):
+    val = np.ascontiguousarray(val, np.float64)
+    _check(lib().mpkx_write_vals(path.encode(), len(val), _p(val)))
+
+
+def read_mtx(path):
+    n = C.c_int()
+    rp, ci, va = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _check(lib().mpkx_read_mtx(path.encode(), C.byref(n), C.byref(rp), C.byref(ci), C.byref(va)))
+    ptr = _take(rp, n.value + 1, np.int32)
+    nnz = int(ptr[-1])
+    return ptr, _take(ci, nnz, np.int32), _take(va, nnz, np.float64)
+
+
+def read_ints(path, n):
+    out = np.zeros(n, np.int32)
+    _check(lib().mpkx_read_ints(path.encode(), n, _p(out), None))
+    return out
+
+
+def write_ints(path, v):
+    _check(lib().mpkx_write_ints(path.encode(), len(v), _p(_i32(v))))
+
+
+read_part = read_ints
+read_level = read_ints
+
+
+# ---- partitions, levels, skirt ----------------------------------------------
+
+def partition_grid(dims, blocks, shift=None):
+    dims, blocks = _i32(dims), _i32(blocks)
+    sh = _i32(shift if shift is not None else np.zeros(len(dims)))
+    out = np.zeros(int(np.prod(dims.astype(np.int64))), np.int32)
+    _check(lib().mpkx_partition_grid(len(dims), _p(dims), _p(blocks), _p(sh), _p(out)))
+    return out
+
+
+def partition_graph(ptr, col, npart):
+    out = np.zeros(len(ptr) - 1, np.int32)
+    _check(lib().mpkx_partition_graph(len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), npart, _p(out)))
+    return out
+
+
+def comp_level(ptr, col, part):
+    """comp_level (mpk2/lib.c:330-371)."""
+    out = np.zeros(len(ptr) - 1, np.int32)
+    _check(lib().mpkx_levels(len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), _p(_i32(part)), None, _p(out)))
+    return out
+
+
+def update_level(ptr, col, part, lorg):
+    """update_level (mpk2/lib.c:373-412)."""
+    out = np.zeros(len(ptr) - 1, np.int32)
+    _check(lib().mpkx_levels(len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), _p(_i32(part)),
+                             _p(_i32(lorg)), _p(out)))
+    return out
+
+
+def level2wcrs(ptr, col, level):
+    we = np.zeros(len(col), np.int32)
+    _check(lib().mpkx_level2wcrs(len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), _p(_i32(level)), _p(we)))
+    return we
+
+
+class Skirt:
+    """Sparse skirt: rows srow[sptr[p]:sptr[p+1]] of partition p, distances sdist."""
+
+    def __init__(self, sptr, srow, sdist, n, nlevel):
+        self.sptr = np.ascontiguousarray(sptr, np.int64)
+        self.srow = _i32(srow)
+        self.sdist = _i32(sdist)
+        self.n = n
+        self.nlevel = nlevel
+
+    @property
+    def npart(self):
+        return len(self.sptr) - 1
+
+    def dense(self):
+        """[npart, n] array of distances, -1 where absent (read_skirt's view)."""
+        d = -np.ones((self.npart, self.n), np.int32)
+        for p in range(self.npart):
+            s = slice(self.sptr[p], self.sptr[p + 1])
+            d[p, self.srow[s]] = self.sdist[s]
+        return d
+
+    def write(self, path):
+        _check(lib().mpkx_write_skirt(path.encode(), self.n, self.npart, self.nlevel,
+                                      _p(self.sptr), _p(self.srow), _p(self.sdist)))
+
+
+def comp_skirt(ptr, col, part, npart, nlevel, lastl=None):
+    """comp_skirt + the write_skirt filter (mpk2/lib.c:584-654)."""
+    L = lib()
+    sp, sr, sd = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    lp = None if lastl is None else _p(_i32(lastl))
+    n = len(ptr) - 1
+    _check(L.mpkx_comp_skirt(n, _p(_i32(ptr)), _p(_i32(col)), _p(_i32(part)), npart, nlevel, lp,
+                             C.byref(sp), C.byref(sr), C.byref(sd)))
+    sptr = _take(sp, npart + 1, np.int64)
+    tot = int(sptr[-1])
+    return Skirt(sptr, _take(sr, tot, np.int32), _take(sd, tot, np.int32), n, nlevel)
+
+
+def read_skirt(path, n, npart, nlevel):
+    sp, sr, sd = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _check(lib().mpkx_read_skirt(path.encode(), n, npart, nlevel, C.byref(sp), C.byref(sr), C.byref(sd)))
+    sptr = _take(sp, npart + 1, np.int64)
+    tot = int(sptr[-1])
+    return Skirt(sptr, _take(sr, tot, np.int32), _take(sd, tot, np.int32), n, nlevel)
+
+
+def lapjv(sums):
+    s = _i32(sums)
+    perm = np.zeros(s.shape[0], np.int32)
+    _check(lib().mpkx_lapjv(_p(s), s.shape[0], _p(perm)))
+    return perm
+
+
+# ---- buffers ----------------------------------------------------------------
+
+class BufferSet:
+    """The exec programs of all partitions (one mpkx_buffers per rank)."""
+
+    def __init__(self, arr_ptr, npart, owned_array=True):
+        self._arr = arr_ptr
+        self.npart = npart
+        self._owned_array = owned_array
+        self.ranks = [arr_ptr[p] for p in range(npart)]
+
+    def __len__(self):
+        return self.npart
+
+    def __getitem__(self, p):
+        return self.ranks[p].contents
+
+    def write(self, dirname):
+        for b in self.ranks:
+            _check(lib().mpkx_write_buffers(b, dirname.encode()))
+
+    def rank_bytes(self, p):
+        """rank%d.bufs bytes of partition p (write_buffers layout)."""
+        with tempfile.TemporaryDirectory() as d:
+            _check(lib().mpkx_write_buffers(self.ranks[p], d.encode()))
+            with open(os.path.join(d, f"rank{p}.bufs"), "rb") as f:
+                return f.read()
+
+    def alloc_val_bufs(self):
+        for b in self.ranks:
+            _check(lib().mpkx_alloc_val_bufs(b))
+
+    def array(self):
+        return C.cast(self._arr, P)
+
+    def free(self):
+        if self._arr is not None:
+            if self._owned_array:
+                lib().mpkx_del_bufs_array(C.cast(self._arr, P), self.npart)
+            else:
+                for b in self.ranks:
+                    lib().mpkx_del_bufs(b)
+            self._arr = None
+            self.ranks = []
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def fill_buffers(ptr, col, val, nlevel, nphase, parts, levels, skirt):
+    """fill_buffers + alloc_fill_mval of every rank (mpk2/buffers.c:573-586).
+
+    parts[ph] are relabelled in place (LAPJV of phases >= 1) as the reference
+    relabels them; returns a BufferSet."""
+    L = lib()
+    parts = [p if (isinstance(p, np.ndarray) and p.dtype == np.int32 and p.flags.c_contiguous)
+             else _i32(p) for p in parts]
+    levels = [_i32(l) for l in levels]
+    pp = (P * len(parts))(*[p.ctypes.data for p in parts])
+    lp = (P * max(len(levels), 1))(*[l.ctypes.data for l in levels])
+    out = C.POINTER(BufPtr)()
+    va = None if val is None else _p(np.ascontiguousarray(val, np.float64))
+    _check(L.mpkx_fill_buffers(len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), va, nlevel, nphase,
+                               C.cast(pp, P), C.cast(lp, P), _p(skirt.sptr), _p(skirt.srow),
+                               _p(skirt.sdist), skirt.npart, C.byref(out)))
+    return BufferSet(out, skirt.npart), parts
+
+
+def read_buffers(dirname, npart):
+    """read_buffers (mpk2/buffers.c:633-697) of ranks 0..npart-1."""
+    arr = (BufPtr * npart)()
+    for r in range(npart):
+        b = lib().mpkx_read_buffers(dirname.encode(), r)
+        if not b:
+            for q in range(r):
+                lib().mpkx_del_bufs(arr[q])
+            raise MpkxError(-3, lib().mpkx_last_error().decode())
+        arr[r] = b
+    bs = BufferSet(arr, npart, owned_array=False)
+    bs._keep = arr
+    return bs
+
+
+def prep_mpk(ptr, col, nlevel, nphase, parts, levels, skirt):
+    """Task lists of mpk2/mpkprep.c (prep_mpk); returns (tcount, tidx)."""
+    L = lib()
+    parts = [_i32(p) for p in parts]
+    levels = [_i32(l) for l in levels]
+    pp = (P * len(parts))(*[p.ctypes.data for p in parts])
+    lp = (P * max(len(levels), 1))(*[l.ctypes.data for l in levels])
+    tcount = np.zeros((nphase + 1) * skirt.npart, np.int32)
+    tidx = C.c_void_p()
+    nt = C.c_long()
+    _check(L.mpkx_prep_mpk(len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), nlevel, nphase,
+                           C.cast(pp, P), C.cast(lp, P), _p(skirt.sptr), _p(skirt.srow),
+                           _p(skirt.sdist), skirt.npart, _p(tcount), C.byref(tidx), C.byref(nt)))
+    return tcount, _take(tidx, nt.value, np.int64)
+
+
+# ---- GPU exec ---------------------------------------------------------------
+
+class Plan:
+    """GPU exec of reference buffers (drop-in for mpk_exec_bufs_val on P ranks)."""
+
+    def __init__(self, bufset):
+        self._h = C.c_void_p()
+        _check(lib().mpkx_plan_create(C.byref(self._h), bufset.array(), bufset.npart))
+
+    def exec_bufs(self, bufset):
+        _check(lib().mpkx_exec_bufs(self._h, bufset.array()))
+
+    def exec(self, x, vv, stream=None):
+        _check(lib().mpkx_exec(self._h, _dev(x), _dev(vv), _stream(stream)))
+
+    def stats(self):
+        r, z, g, l = C.c_long(), C.c_long(), C.c_long(), C.c_int()
+        _check(lib().mpkx_plan_stats(self._h, C.byref(r), C.byref(z), C.byref(g), C.byref(l)))
+        return {"rows": r.value, "nnz": z.value, "ghosts": g.value, "launches": l.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_plan_destroy(self._h)
+            self._h = None
+
+
+class CsrMatrix:
+    """Device CSR matrix for the plain k x SpMV baseline."""
+
+    def __init__(self, ptr, col, val):
+        self._h = C.c_void_p()
+        self.n = len(ptr) - 1
+        self.nnz = int(ptr[-1])
+        _check(lib().mpkx_csr_create(C.byref(self._h), self.n, _p(_i32(ptr)), _p(_i32(col)),
+                                     _p(np.ascontiguousarray(val, np.float64))))
+
+    def spmv_k(self, vv, k, stream=None, use_graph=True):
+        _check(lib().mpkx_spmv_k(self._h, _dev(vv), k, _stream(stream), int(use_graph)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_csr_destroy(self._h)
+            self._h = None
+
+
+class FusedMPK:
+    """Fused skirt-round MPK: s powers per launch, tiles from `part`."""
+
+    EXPLICIT = 1
+
+    def __init__(self, ptr, col, val, part, nlevel, s, flags=0):
+        self._h = C.c_void_p()
+        part = _i32(part)
+        self.npart = int(part.max()) + 1
+        self.n = len(ptr) - 1
+        self.nlevel = nlevel
+        self.s = s
+        _check(lib().mpkx_fused_create(C.byref(self._h), self.n, _p(_i32(ptr)), _p(_i32(col)),
+                                       _p(np.ascontiguousarray(val, np.float64)), _p(part),
+                                       self.npart, nlevel, s, flags))
+
+    def exec(self, x, vv, stream=None, use_graph=True):
+        _check(lib().mpkx_fused_exec(self._h, _dev(x), _dev(vv), _stream(stream), int(use_graph)))
+
+    def round(self, vv, r, stream=None):
+        """Launches round r only (levels r*s+1 .. r*s+w)."""
+        _check(lib().mpkx_fused_round(self._h, _dev(vv), r, _stream(stream)))
+
+    def stats(self):
+        ms, sl, rl, mb = C.c_long(), C.c_long(), C.c_long(), C.c_long()
+        codec, la = C.c_int(), C.c_int()
+        _check(lib().mpkx_fused_stats(self._h, C.byref(ms), C.byref(sl), C.byref(rl), C.byref(mb),
+                                      C.byref(codec), C.byref(la)))
+        return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
+                "matrix_bytes": mb.value, "codec": ["explicit", "dict"][codec.value],
+                "launches": la.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_fused_destroy(self._h)
+            self._h = None
+
+
+def prepare(ptr, col, val, nlevel, nphase, parts):
+    """Levels, skirt and buffers of the mpk2 prep (driver.sh + mpi2_mpkwrtbufs)
+    for given partitions parts[0..max(nphase,1)); returns (levels, skirt, bufset, parts)."""
+    levels = []
+    prev = None
+    for ph in range(nphase):
+        lv = comp_level(ptr, col, parts[ph]) if prev is None else update_level(ptr, col, parts[ph], prev)
+        levels.append(lv)
+        prev = lv
+    npart = int(np.max(parts[0])) + 1
+    sk = comp_skirt(ptr, col, parts[0], npart, nlevel, levels[-1] if nphase else None)
+    bufs, parts = fill_buffers(ptr, col, val, nlevel, nphase, [p.copy() for p in parts], levels, sk)
+    return levels, sk, bufs, parts

@@ -1,0 +1,441 @@
+// dev_exec.hip -- GPU exec of reference buffers, and the plain k x SpMV.
+//
+// Reference exec (mpk2/mpi2_mpkexecbufs_val.c:13-56), one MPI rank per
+// partition:
+//   do_task(0); for ph = 1..nphase { do_comm(ph); do_task(ph); }
+// Here every partition's vv_buf lives in one flat HBM array.  do_task of
+// all partitions runs as one launch per (phase, level) -- rows of a phase
+// are level-major and a level-l row reads only level-(l-1) slots -- and
+// do_comm (gather into vv_sbuf + MPI_Alltoallv into rbuf) becomes one copy
+// kernel moving every ghost value directly between partitions' slots.
+//
+// Row kernel: CSR-stream.  A workgroup takes a run of rows whose entries
+// fit in LDS, forms all products a_ij * x_j with coalesced loads, then each
+// thread sums its row's products in CSR order.  Products and sums are
+// rounded separately (no fused multiply-add), the same operation sequence
+// as the reference's `tmp += mval[mj] * vv_buf[mcol[mj]]`, so results are
+// bitwise equal to the reference and to verify_val.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <unordered_set>
+
+#include "dev_common.h"
+
+using namespace mpkx;
+
+namespace {
+
+constexpr int CS_THREADS = 256;
+constexpr int CS_CAP = 2048;  // products staged per workgroup (16 KB LDS)
+
+template <typename PtrT, bool INDIRECT>
+__global__ __launch_bounds__(CS_THREADS) void k_csr_stream(
+    const PtrT *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ va,
+    const double *x, double *y, const int *__restrict__ out_row, const int *__restrict__ blk,
+    int blk0) {
+  __shared__ double prod[CS_CAP];
+  const int b = blk0 + blockIdx.x;
+  const int r0 = blk[b], r1 = blk[b + 1];
+  const PtrT j0 = rp[r0], j1 = rp[r1];
+  const long len = (long)(j1 - j0);
+  if (len <= CS_CAP) {
+    for (long t = threadIdx.x; t < len; t += CS_THREADS) {
+      const PtrT j = j0 + (PtrT)t;
+      prod[t] = __dmul_rn(va[j], x[ci[j]]);
+    }
+    __syncthreads();
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += CS_THREADS) {
+      double s = 0.0;
+      const PtrT e = rp[r + 1];
+      for (PtrT j = rp[r]; j < e; j++) s = __dadd_rn(s, prod[j - j0]);
+      y[INDIRECT ? out_row[r] : r] = s;
+    }
+  } else if (threadIdx.x == 0) {  // a single row longer than the LDS stage
+    double s = 0.0;
+    for (PtrT j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(va[j], x[ci[j]]));
+    y[INDIRECT ? out_row[r0] : r0] = s;
+  }
+}
+
+__global__ void k_copy_pairs(double *flat, const int *__restrict__ dst, const int *__restrict__ src,
+                             long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) flat[dst[t]] = flat[src[t]];
+}
+
+__global__ void k_load_pairs(double *__restrict__ out, const double *__restrict__ in,
+                             const int *__restrict__ dst, const int *__restrict__ src, long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[dst[t]] = in[src[t]];
+}
+
+__global__ void k_gather(double *__restrict__ out, const double *__restrict__ in,
+                         const int *__restrict__ idx, long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[t] = in[idx[t]];
+}
+
+__global__ void k_scatter(double *__restrict__ out, const double *__restrict__ in,
+                          const int *__restrict__ idx, long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[idx[t]] = in[t];
+}
+
+// Splits rows [r0, r1) into CSR-stream workgroups: consecutive rows while
+// their entries fit in CS_CAP (a longer row gets a workgroup of its own).
+template <typename PtrT>
+void make_blocks(const std::vector<PtrT> &rp, int r0, int r1, std::vector<int> &blk) {
+  int r = r0;
+  while (r < r1) {
+    blk.push_back(r);
+    PtrT start = rp[r];
+    int e = r + 1;
+    if (rp[e] - start <= CS_CAP)
+      while (e < r1 && rp[e + 1] - start <= CS_CAP && e - r < 4 * CS_THREADS) e++;
+    r = e;
+  }
+}
+
+inline unsigned grid1(long count, int threads) { return (unsigned)((count + threads - 1) / threads); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plan for reference buffers
+// ---------------------------------------------------------------------------
+struct mpkx_plan {
+  int n = 0, npart = 0, nlevel = 0, nphase = 0;
+  long flat = 0;                  // total vv_buf slots
+  std::vector<long> base;         // flat offset of each partition's vv_buf
+  std::vector<int> buf_count;
+  // per (phase, level) launch groups
+  struct Group {
+    int ph, level, blk0, nblk;
+  };
+  std::vector<Group> groups;
+  std::vector<long> gather_begin;  // per phase: range in gather pairs
+  std::vector<char> gather_staged; // per phase: sources overlap destinations
+  DevArray<long> rp;
+  DevArray<int> ci, out_row, blk, gdst, gsrc, xdst, xsrc, csrc;
+  DevArray<double> va, vv, stage;
+  long rows = 0, nnz = 0, ghosts = 0;
+};
+
+extern "C" {
+
+int mpkx_plan_create(mpkx_plan **out, mpkx_buffers *const *bufs, int npart) {
+  clear_error();
+  if (!out || !bufs || npart <= 0) return fail(MPKX_EINVAL, "plan_create: bad arguments");
+  const mpkx_buffers *b0 = bufs[0];
+  for (int p = 0; p < npart; p++) {
+    const mpkx_buffers *b = bufs[p];
+    if (!b || b->rank != p || b->npart != npart || b->n != b0->n || b->nlevel != b0->nlevel ||
+        b->nphase != b0->nphase)
+      return fail(MPKX_EINVAL, "plan_create: buffers of rank %d inconsistent", p);
+    if (!b->mval_buf) return fail(MPKX_EINVAL, "plan_create: rank %d has no mval_buf", p);
+  }
+  auto *pl = new (std::nothrow) mpkx_plan;
+  if (!pl) return fail(MPKX_ENOMEM, "plan_create");
+  std::unique_ptr<mpkx_plan> guard(pl);
+  pl->n = b0->n;
+  pl->npart = npart;
+  pl->nlevel = b0->nlevel;
+  pl->nphase = b0->nphase;
+  const int n = pl->n, nph = pl->nphase;
+  pl->base.resize(npart + 1, 0);
+  for (int p = 0; p < npart; p++) {
+    pl->base[p + 1] = pl->base[p] + bufs[p]->buf_count;
+    pl->buf_count.push_back(bufs[p]->buf_count);
+  }
+  pl->flat = pl->base[npart];
+  if (pl->flat >= INT32_MAX) return fail(MPKX_EUNSUPPORTED, "plan_create: %ld slots exceed int32 indexing", pl->flat);
+
+  // Rows grouped by (phase, level); inside a group by partition then row.
+  std::vector<long> rp(1, 0);
+  std::vector<int> ci, outr, blk;
+  std::vector<double> va;
+  try {
+    for (int ph = 0; ph <= nph; ph++) {
+      // level ranges of every partition's mbuf in this phase
+      int lmin = INT32_MAX, lmax = -1;
+      for (int p = 0; p < npart; p++) {
+        const mpkx_buffers *b = bufs[p];
+        const long *im = b->idx_buf + b->mbuf_offsets[ph];
+        for (int m = 0; m < b->mcount[ph]; m++) {
+          int l = (int)(im[m] / n);
+          lmin = std::min(lmin, l);
+          lmax = std::max(lmax, l);
+          if (m > 0 && l < im[m - 1] / n)
+            return fail(MPKX_EINVAL, "plan_create: rank %d phase %d rows not level-major", p, ph);
+        }
+      }
+      if (lmax < 0) continue;
+      std::vector<int> cursor(npart, 0);
+      for (int l = lmin; l <= lmax; l++) {
+        int r0 = (int)rp.size() - 1;
+        for (int p = 0; p < npart; p++) {
+          const mpkx_buffers *b = bufs[p];
+          const long *im = b->idx_buf + b->mbuf_offsets[ph];
+          const long *mp = b->mptr_buf + b->mptr_offsets[ph];
+          const long *mc = b->mcol_buf + b->mcol_offsets[ph];
+          const double *mv = b->mval_buf + b->mcol_offsets[ph];
+          int &m = cursor[p];
+          for (; m < b->mcount[ph] && im[m] / n == l; m++) {
+            for (long j = mp[m]; j < mp[m + 1]; j++) {
+              if (mc[j] < 0 || mc[j] >= b->buf_count)
+                return fail(MPKX_EINVAL, "plan_create: rank %d mcol out of range", p);
+              ci.push_back((int)(pl->base[p] + mc[j]));
+              va.push_back(mv[j]);
+            }
+            rp.push_back((long)ci.size());
+            outr.push_back((int)(pl->base[p] + b->mbuf_offsets[ph] + m));
+          }
+        }
+        int r1 = (int)rp.size() - 1;
+        if (r1 == r0) continue;
+        int blk0 = (int)blk.size();
+        make_blocks(rp, r0, r1, blk);
+        pl->groups.push_back({ph, l, blk0, (int)blk.size() - blk0});
+      }
+    }
+    blk.push_back((int)rp.size() - 1);
+  } catch (const std::bad_alloc &) {
+    return fail(MPKX_ENOMEM, "plan_create: host allocation");
+  }
+  pl->rows = (long)rp.size() - 1;
+  pl->nnz = (long)ci.size();
+
+  // Ghost gather of every phase >= 1: the Alltoallv of do_comm
+  // (mpi2_mpkexecbufs_val.c:32-48) as (destination slot, source slot) pairs.
+  std::vector<int> gdst, gsrc;
+  pl->gather_begin.assign(nph + 2, 0);
+  pl->gather_staged.assign(nph + 1, 0);
+  for (int ph = 1; ph <= nph; ph++) {
+    pl->gather_begin[ph] = (long)gdst.size();
+    for (int dst = 0; dst < npart; dst++) {
+      const mpkx_buffers *bd = bufs[dst];
+      for (int src = 0; src < npart; src++) {
+        const mpkx_buffers *bs = bufs[src];
+        int cnt = bd->recvcounts[ph * npart + src];
+        if (cnt != bs->sendcounts[ph * npart + dst])
+          return fail(MPKX_EINVAL, "plan_create: phase %d send/recv count mismatch %d->%d", ph, src, dst);
+        long dbase = pl->base[dst] + bd->rbuf_offsets[ph] + bd->rdispls[ph * npart + src];
+        const long *si = bs->idx_sbuf + bs->sbuf_offsets[ph] + bs->sdispls[ph * npart + dst];
+        for (int r = 0; r < cnt; r++) {
+          if (si[r] < 0 || si[r] >= bs->buf_count)
+            return fail(MPKX_EINVAL, "plan_create: rank %d idx_sbuf out of range", src);
+          gdst.push_back((int)(dbase + r));
+          gsrc.push_back((int)(pl->base[src] + si[r]));
+        }
+      }
+    }
+    // A source that is also written in the same exchange must be read
+    // before any write: stage through a separate array then.
+    std::unordered_set<int> dset(gdst.begin() + pl->gather_begin[ph], gdst.end());
+    for (size_t t = pl->gather_begin[ph]; t < gsrc.size(); t++)
+      if (dset.count(gsrc[t])) {
+        pl->gather_staged[ph] = 1;
+        break;
+      }
+  }
+  pl->gather_begin[nph + 1] = (long)gdst.size();
+  pl->ghosts = (long)gdst.size();
+
+  // Input load (rbuf[0] = x[idx_buf], mpi2_mpkexecbufs_val.c:86-87) and
+  // result collection (collect_results, mpi2_mpkexec.c:122-146: later
+  // ranks overwrite earlier ones; all copies of a value are equal).
+  std::vector<int> xdst, xsrc, csrc((size_t)n * (pl->nlevel + 1), -1);
+  for (int p = 0; p < npart; p++) {
+    const mpkx_buffers *b = bufs[p];
+    for (int i = 0; i < b->rcount[0]; i++) {
+      if (b->idx_buf[i] < 0 || b->idx_buf[i] >= n)
+        return fail(MPKX_EINVAL, "plan_create: rank %d rbuf[0] holds a non level-0 index", p);
+      xdst.push_back((int)(pl->base[p] + i));
+      xsrc.push_back((int)b->idx_buf[i]);
+    }
+    for (int i = 0; i < b->buf_count; i++) {
+      long v = b->idx_buf[i];
+      if (v < 0 || v >= (long)csrc.size()) return fail(MPKX_EINVAL, "plan_create: idx_buf out of range");
+      csrc[v] = (int)(pl->base[p] + i);
+    }
+  }
+  for (size_t v = 0; v < csrc.size(); v++)
+    if (csrc[v] < 0)
+      return fail(MPKX_ESTRUCT, "plan_create: level %zu row %zu is computed by no partition", v / n, v % n);
+
+  int rc;
+  if ((rc = pl->rp.upload(rp)) || (rc = pl->ci.upload(ci)) || (rc = pl->va.upload(va)) ||
+      (rc = pl->out_row.upload(outr)) || (rc = pl->blk.upload(blk)) || (rc = pl->gdst.upload(gdst)) ||
+      (rc = pl->gsrc.upload(gsrc)) || (rc = pl->xdst.upload(xdst)) || (rc = pl->xsrc.upload(xsrc)) ||
+      (rc = pl->csrc.upload(csrc)) || (rc = pl->vv.alloc(pl->flat)) ||
+      (rc = pl->stage.alloc(std::max<long>(pl->ghosts, 1))))
+    return rc;
+  *out = guard.release();
+  return MPKX_OK;
+}
+
+void mpkx_plan_destroy(mpkx_plan *plan) { delete plan; }
+
+int mpkx_plan_stats(const mpkx_plan *pl, long *rows, long *nnz, long *ghosts, int *launches) {
+  clear_error();
+  if (!pl) return fail(MPKX_EINVAL, "plan_stats: null plan");
+  if (rows) *rows = pl->rows;
+  if (nnz) *nnz = pl->nnz;
+  if (ghosts) *ghosts = pl->ghosts;
+  if (launches) {
+    int l = (int)pl->groups.size();
+    for (int ph = 1; ph <= pl->nphase; ph++)
+      if (pl->gather_begin[ph + 1] > pl->gather_begin[ph]) l += pl->gather_staged[ph] ? 2 : 1;
+    *launches = l;
+  }
+  return MPKX_OK;
+}
+
+}  // extern "C"
+
+// Phases: do_task(0); then (ghost gather, do_task) per phase.
+static int run_phases(mpkx_plan *pl, hipStream_t st) {
+  size_t gi = 0;
+  for (int ph = 0; ph <= pl->nphase; ph++) {
+    if (ph > 0) {
+      long g0 = pl->gather_begin[ph], cnt = pl->gather_begin[ph + 1] - g0;
+      if (cnt > 0) {
+        if (!pl->gather_staged[ph]) {
+          hipLaunchKernelGGL(k_copy_pairs, dim3(grid1(cnt, 256)), dim3(256), 0, st, pl->vv.p,
+                             pl->gdst.p + g0, pl->gsrc.p + g0, cnt);
+        } else {
+          hipLaunchKernelGGL(k_gather, dim3(grid1(cnt, 256)), dim3(256), 0, st, pl->stage.p,
+                             pl->vv.p, pl->gsrc.p + g0, cnt);
+          hipLaunchKernelGGL(k_scatter, dim3(grid1(cnt, 256)), dim3(256), 0, st, pl->vv.p,
+                             pl->stage.p, pl->gdst.p + g0, cnt);
+        }
+      }
+    }
+    for (; gi < pl->groups.size() && pl->groups[gi].ph == ph; gi++) {
+      const auto &g = pl->groups[gi];
+      hipLaunchKernelGGL((k_csr_stream<long, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
+                         pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
+    }
+  }
+  MPKX_HIP(hipGetLastError());
+  return MPKX_OK;
+}
+
+extern "C" {
+
+int mpkx_exec_bufs(mpkx_plan *pl, mpkx_buffers *const *bufs) {
+  clear_error();
+  if (!pl || !bufs) return fail(MPKX_EINVAL, "exec_bufs: bad arguments");
+  for (int p = 0; p < pl->npart; p++)
+    if (!bufs[p] || !bufs[p]->vv_buf || bufs[p]->buf_count != pl->buf_count[p])
+      return fail(MPKX_EINVAL, "exec_bufs: rank %d buffers do not match the plan", p);
+  for (int p = 0; p < pl->npart; p++)
+    MPKX_HIP(hipMemcpy(pl->vv.p + pl->base[p], bufs[p]->vv_buf, sizeof(double) * bufs[p]->rcount[0],
+                       hipMemcpyHostToDevice));
+  int rc = run_phases(pl, nullptr);
+  if (rc) return rc;
+  MPKX_HIP(hipDeviceSynchronize());
+  for (int p = 0; p < pl->npart; p++)
+    MPKX_HIP(hipMemcpy(bufs[p]->vv_buf, pl->vv.p + pl->base[p], sizeof(double) * pl->buf_count[p],
+                       hipMemcpyDeviceToHost));
+  return MPKX_OK;
+}
+
+int mpkx_exec(mpkx_plan *pl, const double *x, double *vv, void *stream) {
+  clear_error();
+  if (!pl || !x || !vv) return fail(MPKX_EINVAL, "exec: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  long nx = (long)pl->xdst.count;
+  // rbuf[0] of every partition = x[idx_buf[i]]
+  if (nx > 0)
+    hipLaunchKernelGGL(k_load_pairs, dim3(grid1(nx, 256)), dim3(256), 0, st, pl->vv.p, x,
+                       pl->xdst.p, pl->xsrc.p, nx);
+  int rc = run_phases(pl, st);
+  if (rc) return rc;
+  long nv = (long)pl->csrc.count;
+  hipLaunchKernelGGL(k_gather, dim3(grid1(nv, 256)), dim3(256), 0, st, vv, pl->vv.p, pl->csrc.p, nv);
+  MPKX_HIP(hipGetLastError());
+  return MPKX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain k x SpMV (mpk2/verify_val.c:44-75)
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct mpkx_csr {
+  int n = 0;
+  long nnz = 0;
+  DevArray<int> rp, ci, blk;
+  DevArray<double> va;
+  int nblk = 0;
+  GraphCache graph;
+};
+
+static void spmv_launches(mpkx_csr *m, double *vv, int k, hipStream_t st) {
+  for (int l = 0; l < k; l++)
+    hipLaunchKernelGGL((k_csr_stream<int, false>), dim3(m->nblk), dim3(CS_THREADS), 0, st, m->rp.p,
+                       m->ci.p, m->va.p, vv + (size_t)l * m->n, vv + (size_t)(l + 1) * m->n,
+                       (const int *)nullptr, m->blk.p, 0);
+}
+
+extern "C" {
+
+int mpkx_csr_create(mpkx_csr **out, int n, const int *ptr, const int *col, const double *val) {
+  clear_error();
+  if (!out || n <= 0 || !ptr || !col || !val) return fail(MPKX_EINVAL, "csr_create: bad arguments");
+  auto *m = new (std::nothrow) mpkx_csr;
+  if (!m) return fail(MPKX_ENOMEM, "csr_create");
+  std::unique_ptr<mpkx_csr> guard(m);
+  m->n = n;
+  m->nnz = ptr[n];
+  std::vector<int> rp(ptr, ptr + n + 1), blk;
+  for (int i = 0; i < n; i++)
+    if (ptr[i + 1] < ptr[i]) return fail(MPKX_EINVAL, "csr_create: row pointer decreases at %d", i);
+  for (long j = 0; j < m->nnz; j++)
+    if (col[j] < 0 || col[j] >= n) return fail(MPKX_EINVAL, "csr_create: column out of range at %ld", j);
+  make_blocks(rp, 0, n, blk);
+  blk.push_back(n);
+  m->nblk = (int)blk.size() - 1;
+  int rc;
+  if ((rc = m->rp.upload(rp)) || (rc = m->ci.upload(col, m->nnz)) || (rc = m->va.upload(val, m->nnz)) ||
+      (rc = m->blk.upload(blk)))
+    return rc;
+  *out = guard.release();
+  return MPKX_OK;
+}
+
+void mpkx_csr_destroy(mpkx_csr *m) { delete m; }
+
+int mpkx_spmv_k(mpkx_csr *m, double *vv, int k, void *stream, int use_graph) {
+  clear_error();
+  if (!m || !vv || k < 0) return fail(MPKX_EINVAL, "spmv_k: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    spmv_launches(m, vv, k, st);
+    MPKX_HIP(hipGetLastError());
+    return MPKX_OK;
+  }
+  if (!m->graph.matches(vv, nullptr, st, k)) {
+    m->graph.reset();
+    hipStream_t cap = st;
+    bool own = false;
+    if (cap == nullptr) {  // the legacy default stream cannot be captured
+      MPKX_HIP(hipStreamCreate(&cap));
+      own = true;
+    }
+    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    spmv_launches(m, vv, k, cap);
+    MPKX_HIP(hipStreamEndCapture(cap, &m->graph.graph));
+    MPKX_HIP(hipGraphInstantiate(&m->graph.exec, m->graph.graph, nullptr, nullptr, 0));
+    if (own) MPKX_HIP(hipStreamDestroy(cap));
+    m->graph.k0 = vv;
+    m->graph.stream = st;
+    m->graph.k = k;
+  }
+  MPKX_HIP(hipGraphLaunch(m->graph.exec, st));
+  return MPKX_OK;
+}
+
+}  // extern "C"

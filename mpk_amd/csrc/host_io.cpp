@@ -1,0 +1,564 @@
+// host_io.cpp -- error state, file formats and generators of libmpkx.
+//
+// Formats follow the reference text/binary layouts so files written here
+// are read by the reference programs and vice versa:
+//   graph text   mpk2/lib.c:69-146 (read_crs / write_crs)
+//   values text  mpk2/gen2.c:69-76 ("%lf "), read by comm_data.c:160-187
+//   int lists    mpk2/lib.c:273-284, :414-434 (part / level files)
+//   skirt text   mpk2/lib.c:617-707
+//   rank%d.bufs  mpk2/buffers.c:588-697
+//   Matrix Market pdmpk/csr.cc:37-121
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace mpkx {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+// Reads a whole text file; returns false when it cannot be opened.
+static bool slurp(const char *path, std::string *out) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return false;
+  std::string s;
+  char buf[1 << 16];
+  size_t r;
+  while ((r = fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, r);
+  fclose(f);
+  *out = std::move(s);
+  return true;
+}
+
+template <class T>
+static T *dup_array(const std::vector<T> &v) {
+  T *p = (T *)malloc(sizeof(T) * (v.size() ? v.size() : 1));
+  if (p && !v.empty()) memcpy(p, v.data(), sizeof(T) * v.size());
+  return p;
+}
+
+}  // namespace mpkx
+
+using namespace mpkx;
+
+extern "C" {
+
+const char *mpkx_last_error(void) { return g_err.c_str(); }
+const char *mpkx_version(void) { return "mpkx 0.1 (gfx950)"; }
+void mpkx_free(void *p) { free(p); }
+
+void mpkx_uniform_pm1(uint64_t seed, long n, double *x) {
+  // SplitMix64: Steele, Lea & Flood, OOPSLA 2014.
+  uint64_t s = seed;
+  for (long i = 0; i < n; i++) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    x[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+// Stencils in mpk2/gen2.c order.  A row's entries are emitted in the order
+// gen2 prints them; `scale` multiplies gen2's values.
+int mpkx_gen2(int type, int size, double scale, int *n_out, long *nnz_out,
+              int *ptr, int *col, double *val) {
+  clear_error();
+  if (size <= 0 || type < 0 || type > 4)
+    return fail(MPKX_EINVAL, "gen2: bad type %d or size %d", type, size);
+  const long N = size;
+  long n = type == 4 ? N * N * N : N * N;
+  if (n > INT32_MAX) return fail(MPKX_EINVAL, "gen2: n too large");
+  long p = 0, r = 0;
+  auto put = [&](long c, double v) {
+    if (col) col[p] = (int)c;
+    if (val) val[p] = v * scale;
+    p++;
+  };
+  if (ptr) ptr[0] = 0;
+  if (type == 4) {
+    for (long i = 0; i < N; i++)
+      for (long j = 0; j < N; j++)
+        for (long k = 0; k < N; k++) {
+          long c = (i * N + j) * N + k;
+          put(c, -6.0);
+          if (i > 0) put(c - N * N, 1.0);
+          if (i < N - 1) put(c + N * N, 1.0);
+          if (j > 0) put(c - N, 1.0);
+          if (j < N - 1) put(c + N, 1.0);
+          if (k > 0) put(c - 1, 1.0);
+          if (k < N - 1) put(c + 1, 1.0);
+          if (ptr) ptr[++r] = (int)p;
+        }
+  } else {
+    for (long i = 0; i < N; i++)
+      for (long j = 0; j < N; j++) {
+        long c = i * N + j;
+        if (type == 0) {  // m5p, gen2.c:59-88
+          if (i > 0) put(c - N, 1.0);
+          if (j > 0) put(c - 1, 1.0);
+          put(c, -4.0);
+          if (j < N - 1) put(c + 1, 1.0);
+          if (i < N - 1) put(c + N, 1.0);
+        } else if (type == 1) {  // t5p (torus), gen2.c:90-137
+          if (i == N - 1) put(j, 1.0);
+          if (i > 0) put(c - N, 1.0);
+          if (j == N - 1) put(i * N, 1.0);
+          if (j > 0) put(c - 1, 1.0);
+          put(c, -4.0);
+          if (i < N - 1) put(c + N, 1.0);
+          if (j < N - 1) put(c + 1, 1.0);
+          if (i == 0) put((N - 1) * N + j, 1.0);
+          if (j == 0) put(i * N + N - 1, 1.0);
+        } else if (type == 2) {  // m7p, gen2.c:139-178
+          if (i > 0 && j > 0) put(c - N - 1, 1.0);
+          if (i > 0) put(c - N, 1.0);
+          if (j > 0) put(c - 1, 1.0);
+          put(c, -6.0);
+          if (j < N - 1) put(c + 1, 1.0);
+          if (i < N - 1) put(c + N, 1.0);
+          if (i < N - 1 && j < N - 1) put(c + N + 1, 1.0);
+        } else {  // m9p, gen2.c:180-227
+          if (i > 0) put(c - N, 1.0);
+          if (i < N - 1) put(c + N, 1.0);
+          if (j > 0) put(c - 1, 1.0);
+          if (j < N - 1) put(c + 1, 1.0);
+          put(c, -8.0);
+          if (i > 0 && j > 0) put(c - N - 1, 1.0);
+          if (i < N - 1 && j < N - 1) put(c + N + 1, 1.0);
+          if (i > 0 && j < N - 1) put(c - N + 1, 1.0);
+          if (i < N - 1 && j > 0) put(c + N - 1, 1.0);
+        }
+        if (ptr) ptr[++r] = (int)p;
+      }
+  }
+  if (p > INT32_MAX) return fail(MPKX_EINVAL, "gen2: nnz too large");
+  if (n_out) *n_out = (int)n;
+  if (nnz_out) *nnz_out = p;
+  return MPKX_OK;
+}
+
+int mpkx_read_crs(const char *path, int *n_out, int **ptr_out, int **col_out) {
+  clear_error();
+  std::string s;
+  if (!slurp(path, &s)) return fail(MPKX_EIO, "cannot open %s", path);
+  const char *c = s.c_str(), *end = c + s.size();
+  const char *eol = (const char *)memchr(c, '\n', end - c);
+  std::string head(c, eol ? eol : end);
+  int nv = 0, ne = 0, p3, p4;
+  int got = sscanf(head.c_str(), "%d %d %d %d", &nv, &ne, &p3, &p4);
+  if (got != 2) return fail(MPKX_EIO, "read_crs: header format not supported: %s", head.c_str());
+  if (nv <= 0 || ne <= 0) return fail(MPKX_EIO, "read_crs: nv=%d ne=%d", nv, ne);
+  std::vector<int> rp(1, 0), cl;
+  MPKX_TRY_ALLOC(rp.reserve((size_t)nv + 1); cl.reserve((size_t)ne * 2));
+  c = eol ? eol + 1 : end;
+  int rows = 0;
+  while (c < end && rows < nv) {
+    const char *le = (const char *)memchr(c, '\n', end - c);
+    if (!le) le = end;
+    const char *q = c;
+    while (q < le) {
+      while (q < le && (unsigned char)*q <= ' ') q++;
+      if (q >= le) break;
+      if (!isdigit((unsigned char)*q) && *q != '-' && *q != '+') break;
+      long v = strtol(q, (char **)&q, 10);
+      if ((long)cl.size() >= 2L * ne)
+        return fail(MPKX_EIO, "read_crs: too many edges in crs (%d %zu)", rows, cl.size());
+      if (v - 1 < 0 || v - 1 >= nv)
+        return fail(MPKX_EIO, "read_crs: error edge %d %ld", rows, v - 1);
+      cl.push_back((int)(v - 1));
+    }
+    rp.push_back((int)cl.size());
+    rows++;
+    c = le + 1;
+  }
+  while (rows < nv) {  // the reference leaves missing trailing rows unset; treat as empty
+    rp.push_back((int)cl.size());
+    rows++;
+  }
+  *n_out = nv;
+  *ptr_out = dup_array(rp);
+  *col_out = dup_array(cl);
+  if (!*ptr_out || !*col_out) return fail(MPKX_ENOMEM, "read_crs: allocation");
+  return MPKX_OK;
+}
+
+int mpkx_write_crs(const char *path, int n, const int *ptr, const int *col) {
+  clear_error();
+  FILE *f = fopen(path, "w");
+  if (!f) return fail(MPKX_EIO, "cannot open %s", path);
+  long loops = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) loops += col[j] == i;
+  fprintf(f, "%d %ld\n", n, ((long)ptr[n] + loops) / 2);
+  std::string line;
+  char tmp[16];
+  for (int i = 0; i < n; i++) {
+    line.clear();
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+      int k = snprintf(tmp, sizeof tmp, "%d ", col[j] + 1);
+      line.append(tmp, k);
+    }
+    line.push_back('\n');
+    fwrite(line.data(), 1, line.size(), f);
+  }
+  fclose(f);
+  return MPKX_OK;
+}
+
+int mpkx_read_vals(const char *path, long nnz, double *val) {
+  clear_error();
+  std::string s;
+  if (!slurp(path, &s)) return fail(MPKX_EIO, "cannot open %s", path);
+  const char *c = s.c_str();
+  for (long i = 0; i < nnz; i++) {
+    char *e;
+    val[i] = strtod(c, &e);
+    if (e == c) return fail(MPKX_EIO, "read_vals: %s has %ld values, %ld expected", path, i, nnz);
+    c = e;
+  }
+  return MPKX_OK;
+}
+
+int mpkx_write_vals(const char *path, long nnz, const double *val) {
+  clear_error();
+  FILE *f = fopen(path, "w");
+  if (!f) return fail(MPKX_EIO, "cannot open %s", path);
+  for (long i = 0; i < nnz; i++) fprintf(f, "%lf ", val[i]);
+  fclose(f);
+  return MPKX_OK;
+}
+
+int mpkx_read_ints(const char *path, int n, int *out, int *npart) {
+  clear_error();
+  std::string s;
+  if (!slurp(path, &s)) return fail(MPKX_EIO, "cannot open %s", path);
+  const char *c = s.c_str();
+  int mx = 0;
+  for (int i = 0; i < n; i++) {
+    char *e;
+    long v = strtol(c, &e, 10);
+    if (e == c) return fail(MPKX_EIO, "%s: %d integers, %d expected", path, i, n);
+    out[i] = (int)v;
+    if (v > mx) mx = (int)v;
+    c = e;
+  }
+  if (npart) *npart = mx + 1;
+  return MPKX_OK;
+}
+
+int mpkx_write_ints(const char *path, int n, const int *v) {
+  clear_error();
+  FILE *f = fopen(path, "w");
+  if (!f) return fail(MPKX_EIO, "cannot open %s", path);
+  for (int i = 0; i < n; i++) fprintf(f, "%d\n", v[i]);
+  fclose(f);
+  return MPKX_OK;
+}
+
+// Matrix Market coordinate reader.  The reference (pdmpk/csr.cc:84-121)
+// appends a mirrored entry for symmetric files but counts only the stored
+// one in ptr; here the mirrored entry is counted too, so ptr matches col.
+int mpkx_read_mtx(const char *path, int *n_out, int **ptr_out, int **col_out,
+                  double **val_out) {
+  clear_error();
+  std::string s;
+  if (!slurp(path, &s)) return fail(MPKX_EIO, "cannot open %s", path);
+  const char *c = s.c_str(), *end = c + s.size();
+  auto next_line = [&](std::string *ln) -> bool {
+    if (c >= end) return false;
+    const char *e = (const char *)memchr(c, '\n', end - c);
+    if (!e) e = end;
+    ln->assign(c, e);
+    c = e + 1;
+    return true;
+  };
+  std::string ln;
+  if (!next_line(&ln)) return fail(MPKX_EIO, "mtx: empty file");
+  char w[5][64] = {{0}};
+  if (sscanf(ln.c_str(), "%63s %63s %63s %63s %63s", w[0], w[1], w[2], w[3], w[4]) != 5 ||
+      strcmp(w[0], "%%MatrixMarket") || strcmp(w[1], "matrix"))
+    return fail(MPKX_EINVAL, "Incorrect mtx banner!");
+  if (strcmp(w[2], "coordinate")) return fail(MPKX_EINVAL, "Not coordinate (sparse) matrix!");
+  bool pattern = !strcmp(w[3], "pattern");
+  if (!pattern && strcmp(w[3], "real")) return fail(MPKX_EINVAL, "Not real or pattern matrix!");
+  bool symmetric = !strcmp(w[4], "symmetric");
+  if (!symmetric && strcmp(w[4], "general")) return fail(MPKX_EINVAL, "Not general or symmetric matrix!");
+  do {
+    if (!next_line(&ln)) return fail(MPKX_EIO, "mtx: no size line");
+  } while (!ln.empty() && ln[0] == '%');
+  long m, n, nz;
+  if (sscanf(ln.c_str(), "%ld %ld %ld", &m, &n, &nz) != 3) return fail(MPKX_EIO, "mtx: bad size line");
+  if (m != n) return fail(MPKX_EINVAL, "Matrix not square");
+  std::vector<std::vector<int>> js(n);
+  std::vector<std::vector<double>> vs(n);
+  while (next_line(&ln)) {
+    if (ln.empty() || ln[0] == '%') continue;
+    long i, j;
+    double v = 1.0;
+    const char *q = ln.c_str();
+    char *e;
+    i = strtol(q, &e, 10);
+    if (e == q) continue;
+    q = e;
+    j = strtol(q, &e, 10);
+    q = e;
+    if (!pattern) v = strtod(q, &e);
+    i--, j--;
+    if (i < 0 || i >= n || j < 0 || j >= n) return fail(MPKX_EIO, "mtx: index out of range");
+    js[i].push_back((int)j);
+    vs[i].push_back(v);
+    if (symmetric && i != j) {
+      js[j].push_back((int)i);
+      vs[j].push_back(v);
+    }
+  }
+  std::vector<int> rp(n + 1, 0), cl;
+  std::vector<double> va;
+  for (long i = 0; i < n; i++) {
+    rp[i + 1] = rp[i] + (int)js[i].size();
+    cl.insert(cl.end(), js[i].begin(), js[i].end());
+    va.insert(va.end(), vs[i].begin(), vs[i].end());
+  }
+  if (pattern)
+    for (long i = 0; i < n; i++) {
+      double d = 1.0 / (rp[i + 1] - rp[i]);
+      for (int t = rp[i]; t < rp[i + 1]; t++) va[t] = d;
+    }
+  *n_out = (int)n;
+  *ptr_out = dup_array(rp);
+  *col_out = dup_array(cl);
+  *val_out = dup_array(va);
+  return MPKX_OK;
+}
+
+int mpkx_write_skirt(const char *path, int n, int npart, int nlevel,
+                     const long *sptr, const int *srow, const int *sdist) {
+  clear_error();
+  FILE *f = fopen(path, "w");
+  if (!f) return fail(MPKX_EIO, "cannot open %s", path);
+  fprintf(f, "skirt %d nodes, %d parts\n", n, npart);
+  for (int p = 0; p < npart; p++) {
+    fprintf(f, "partition %d\n", p);
+    for (long t = sptr[p]; t < sptr[p + 1]; t++) fprintf(f, "%d %d\n", srow[t], nlevel - sdist[t]);
+  }
+  fclose(f);
+  return MPKX_OK;
+}
+
+int mpkx_read_skirt(const char *path, int n, int npart, int nlevel, long **sptr,
+                    int **srow, int **sdist) {
+  clear_error();
+  std::string s;
+  if (!slurp(path, &s)) return fail(MPKX_EIO, "cannot open %s", path);
+  const char *c = s.c_str(), *end = c + s.size();
+  int hn = -1, hp = -1;
+  if (sscanf(c, "skirt %d nodes, %d parts", &hn, &hp) != 2 || hn != n || hp != npart)
+    return fail(MPKX_EIO, "error in reading skirt: %d/%d %d/%d", hn, n, hp, npart);
+  std::vector<long> sp(npart + 1, 0);
+  std::vector<int> sr, sd;
+  std::vector<std::vector<std::pair<int, int>>> per(npart);
+  int cur = -1;
+  c = (const char *)memchr(c, '\n', end - c);
+  c = c ? c + 1 : end;
+  while (c < end) {
+    const char *le = (const char *)memchr(c, '\n', end - c);
+    if (!le) le = end;
+    if (*c == 'p') {
+      if (sscanf(c, "partition %d", &cur) != 1 || cur < 0 || cur >= npart)
+        return fail(MPKX_EIO, "error in reading skirt: partition line");
+    } else if (le > c) {
+      int k, m;
+      if (sscanf(c, "%d %d", &k, &m) != 2 || k < 0 || k >= n || m < 0 || m > nlevel || cur < 0)
+        return fail(MPKX_EIO, "error in reading skirt: bad entry");
+      per[cur].push_back({k, nlevel - m});
+    }
+    c = le + 1;
+  }
+  for (int p = 0; p < npart; p++) {
+    std::sort(per[p].begin(), per[p].end());
+    sp[p + 1] = sp[p] + (long)per[p].size();
+    for (auto &e : per[p]) {
+      sr.push_back(e.first);
+      sd.push_back(e.second);
+    }
+  }
+  *sptr = dup_array(sp);
+  *srow = dup_array(sr);
+  *sdist = dup_array(sd);
+  return MPKX_OK;
+}
+
+// ---- rank%d.bufs ----------------------------------------------------------
+
+static void bufs_alloc0(mpkx_buffers *b) {
+  int a1 = b->npart * (b->nphase + 1), a2 = b->nphase + 1;
+  b->recvcounts = (int *)calloc(a1, sizeof(int));
+  b->sendcounts = (int *)calloc(a1, sizeof(int));
+  b->rdispls = (int *)calloc(a1, sizeof(int));
+  b->sdispls = (int *)calloc(a1, sizeof(int));
+  b->rcount = (int *)calloc(a2, sizeof(int));
+  b->mcount = (int *)calloc(a2, sizeof(int));
+  b->scount = (int *)calloc(a2, sizeof(int));
+  b->rbuf_offsets = (int *)calloc(a2, sizeof(int));
+  b->mbuf_offsets = (int *)calloc(a2, sizeof(int));
+  b->sbuf_offsets = (int *)calloc(a2, sizeof(int));
+  b->mptr_offsets = (int *)calloc(a2, sizeof(int));
+  b->mcol_offsets = (int *)calloc(a2, sizeof(int));
+}
+
+mpkx_buffers *mpkx_new_bufs_internal(int n, int npart, int nlevel, int nphase, int rank) {
+  mpkx_buffers *b = (mpkx_buffers *)calloc(1, sizeof(mpkx_buffers));
+  if (!b) return nullptr;
+  b->n = n;
+  b->npart = npart;
+  b->nlevel = nlevel;
+  b->nphase = nphase;
+  b->rank = rank;
+  bufs_alloc0(b);
+  return b;
+}
+
+mpkx_buffers *mpkx_read_buffers(const char *dir, int rank) {
+  clear_error();
+  std::string fname = std::string(dir) + "/rank" + std::to_string(rank) + ".bufs";
+  FILE *f = fopen(fname.c_str(), "rb");
+  if (!f) {
+    fail(MPKX_EIO, "cannot open %s", fname.c_str());
+    return nullptr;
+  }
+  int h[9];
+  if (fread(h, sizeof(int), 9, f) != 9 || h[1] <= 0 || h[3] < 0) {
+    fclose(f);
+    fail(MPKX_EIO, "%s: short or bad header", fname.c_str());
+    return nullptr;
+  }
+  mpkx_buffers *b = mpkx_new_bufs_internal(h[0], h[1], h[2], h[3], h[4]);
+  b->buf_count = h[5];
+  b->buf_scount = h[6];
+  b->mptr_count = h[7];
+  b->mcol_count = h[8];
+  size_t a1 = (size_t)b->npart * (b->nphase + 1), a2 = b->nphase + 1;
+  bool ok = fread(b->recvcounts, 4, a1, f) == a1 && fread(b->sendcounts, 4, a1, f) == a1 &&
+            fread(b->rdispls, 4, a1, f) == a1 && fread(b->sdispls, 4, a1, f) == a1 &&
+            fread(b->rcount, 4, a2, f) == a2 && fread(b->mcount, 4, a2, f) == a2 &&
+            fread(b->scount, 4, a2, f) == a2 && fread(b->rbuf_offsets, 4, a2, f) == a2 &&
+            fread(b->mbuf_offsets, 4, a2, f) == a2 && fread(b->sbuf_offsets, 4, a2, f) == a2 &&
+            fread(b->mptr_offsets, 4, a2, f) == a2 && fread(b->mcol_offsets, 4, a2, f) == a2;
+  b->idx_buf = (long *)malloc(8 * ((size_t)b->buf_count + 1));
+  b->idx_sbuf = (long *)malloc(8 * ((size_t)b->buf_scount + 1));
+  b->mptr_buf = (long *)malloc(8 * ((size_t)b->mptr_count + 1));
+  b->mcol_buf = (long *)malloc(8 * ((size_t)b->mcol_count + 1));
+  ok = ok && b->idx_buf && b->idx_sbuf && b->mptr_buf && b->mcol_buf &&
+       fread(b->idx_buf, 8, b->buf_count, f) == (size_t)b->buf_count &&
+       fread(b->idx_sbuf, 8, b->buf_scount, f) == (size_t)b->buf_scount &&
+       fread(b->mptr_buf, 8, b->mptr_count, f) == (size_t)b->mptr_count &&
+       fread(b->mcol_buf, 8, b->mcol_count, f) == (size_t)b->mcol_count;
+  if (!ok) {
+    fclose(f);
+    mpkx_del_bufs(b);
+    fail(MPKX_EIO, "%s: truncated", fname.c_str());
+    return nullptr;
+  }
+  // mval_buf is present only if the file holds it (buffers.c:683-690).
+  b->mval_buf = (double *)malloc(8 * ((size_t)b->mcol_count + 1));
+  if (fread(b->mval_buf, 8, b->mcol_count, f) < (size_t)b->mcol_count) {
+    free(b->mval_buf);
+    b->mval_buf = nullptr;
+  }
+  fclose(f);
+  return b;
+}
+
+int mpkx_write_buffers(const mpkx_buffers *b, const char *dir) {
+  clear_error();
+  std::string fname = std::string(dir) + "/rank" + std::to_string(b->rank) + ".bufs";
+  FILE *f = fopen(fname.c_str(), "wb");
+  if (!f) return fail(MPKX_EIO, "cannot open %s", fname.c_str());
+  size_t a1 = (size_t)b->npart * (b->nphase + 1), a2 = b->nphase + 1;
+  int h[9] = {b->n, b->npart, b->nlevel, b->nphase, b->rank,
+              b->buf_count, b->buf_scount, b->mptr_count, b->mcol_count};
+  fwrite(h, 4, 9, f);
+  fwrite(b->recvcounts, 4, a1, f);
+  fwrite(b->sendcounts, 4, a1, f);
+  fwrite(b->rdispls, 4, a1, f);
+  fwrite(b->sdispls, 4, a1, f);
+  fwrite(b->rcount, 4, a2, f);
+  fwrite(b->mcount, 4, a2, f);
+  fwrite(b->scount, 4, a2, f);
+  fwrite(b->rbuf_offsets, 4, a2, f);
+  fwrite(b->mbuf_offsets, 4, a2, f);
+  fwrite(b->sbuf_offsets, 4, a2, f);
+  fwrite(b->mptr_offsets, 4, a2, f);
+  fwrite(b->mcol_offsets, 4, a2, f);
+  fwrite(b->idx_buf, 8, b->buf_count, f);
+  fwrite(b->idx_sbuf, 8, b->buf_scount, f);
+  fwrite(b->mptr_buf, 8, b->mptr_count, f);
+  fwrite(b->mcol_buf, 8, b->mcol_count, f);
+  if (b->mval_buf) fwrite(b->mval_buf, 8, b->mcol_count, f);
+  bool bad = ferror(f);
+  fclose(f);
+  return bad ? fail(MPKX_EIO, "write error on %s", fname.c_str()) : MPKX_OK;
+}
+
+int mpkx_alloc_val_bufs(mpkx_buffers *b) {
+  clear_error();
+  free(b->vv_buf);
+  free(b->vv_sbuf);
+  b->vv_buf = (double *)calloc((size_t)b->buf_count + 1, sizeof(double));
+  b->vv_sbuf = (double *)calloc((size_t)b->buf_scount + 1, sizeof(double));
+  if (!b->vv_buf || !b->vv_sbuf) return fail(MPKX_ENOMEM, "alloc_val_bufs");
+  return MPKX_OK;
+}
+
+void mpkx_del_bufs(mpkx_buffers *b) {
+  if (!b) return;
+  free(b->recvcounts);
+  free(b->sendcounts);
+  free(b->rdispls);
+  free(b->sdispls);
+  free(b->rcount);
+  free(b->mcount);
+  free(b->scount);
+  free(b->rbuf_offsets);
+  free(b->mbuf_offsets);
+  free(b->sbuf_offsets);
+  free(b->mptr_offsets);
+  free(b->mcol_offsets);
+  free(b->idx_buf);
+  free(b->idx_sbuf);
+  free(b->mptr_buf);
+  free(b->mcol_buf);
+  free(b->vv_buf);
+  free(b->vv_sbuf);
+  free(b->mval_buf);
+  free(b);
+}
+
+void mpkx_del_bufs_array(mpkx_buffers **bufs, int npart) {
+  if (!bufs) return;
+  for (int p = 0; p < npart; p++) mpkx_del_bufs(bufs[p]);
+  free(bufs);
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// internal.h -- shared helpers of libmpkx (not part of the C ABI).
+#pragma once
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/mpkx.h"
+
+namespace mpkx {
+
+// Records the message for mpkx_last_error() and returns `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Runs body(i) for i in [0, count) on up to `threads` host threads (0 =
+// hardware concurrency, capped at 16).  Work is handed out dynamically.
+template <class F>
+void parallel_for(long count, F body, int threads = 0) {
+  if (threads <= 0) {
+    threads = (int)std::thread::hardware_concurrency();
+    if (threads > 16) threads = 16;
+    if (threads < 1) threads = 1;
+  }
+  if (count <= 1 || threads == 1) {
+    for (long i = 0; i < count; i++) body(i);
+    return;
+  }
+  if (threads > count) threads = (int)count;
+  std::atomic<long> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; t++)
+    pool.emplace_back([&]() {
+      for (;;) {
+        long i = next.fetch_add(1);
+        if (i >= count) break;
+        body(i);
+      }
+    });
+  for (auto &th : pool) th.join();
+}
+
+// Sparse skirt (rows of partition p: srow[sptr[p]..sptr[p+1]) ascending).
+struct SkirtView {
+  const long *sptr;
+  const int *srow;
+  const int *sdist;
+};
+
+}  // namespace mpkx
+
+#define MPKX_TRY_ALLOC(expr)                                            \
+  try {                                                                 \
+    expr;                                                               \
+  } catch (const std::bad_alloc &) {                                    \
+    return mpkx::fail(MPKX_ENOMEM, "host allocation failed: %s", #expr); \
+  }

@@ -1,0 +1,153 @@
+"""GPU parity: every HIP path against the oracle / reference outputs.
+
+FP64 paths compute products and sums separately in CSR order, so the bar is
+bitwise equality (which implies the <= 1e-13 normwise per-power bound)."""
+import numpy as np
+import pytest
+
+import mpk_amd as M
+import tests.oracle_api as O
+from tests.gpu_util import say, to_dev, to_host, zeros_dev
+
+pytestmark = pytest.mark.gpu
+CASES = O.cases()
+
+
+def per_power_rel_err(vv, ref, n, k):
+    out = []
+    for l in range(k + 1):
+        a, b = vv[l * n:(l + 1) * n], ref[l * n:(l + 1) * n]
+        out.append(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_exact_plan_bitwise_reference(case):
+    d, rp, ci, va, parts, levs = O.case_inputs(case)
+    say("exact plan", case)
+    levels, sk, bufs, _ = M.prepare(rp, ci, va, d["nlevel"], d["nphase"], parts)
+    plan = M.Plan(bufs)
+    x = O.uniform(d["seed"], d["n"])
+    vv = zeros_dev(d["n"] * (d["nlevel"] + 1))
+    plan.exec(to_dev(x), vv)
+    out = to_host(vv)
+    assert O.sha(out.tobytes()) == d["results_seeded.bin"]
+    # host drop-in: each rank's vv_buf[0:rcount[0]] loaded as the reference main does
+    bufs.alloc_val_bufs()
+    for p in range(d["npart"]):
+        b = bufs[p]
+        for i in range(b.rcount[0]):
+            b.vv_buf[i] = x[b.idx_buf[i]]
+    plan.exec_bufs(bufs)
+    vv2 = np.zeros(d["n"] * (d["nlevel"] + 1))
+    for p in range(d["npart"]):
+        b = bufs[p]
+        idx = np.ctypeslib.as_array(b.idx_buf, (b.buf_count,))
+        vals = np.ctypeslib.as_array(b.vv_buf, (b.buf_count,))
+        vv2[idx] = vals
+    assert O.sha(vv2.tobytes()) == d["results_seeded.bin"]
+
+
+@pytest.mark.parametrize("typ,size,k", [("m5p", 64, 4), ("cube", 32, 8), ("m9p", 50, 16), ("t5p", 40, 3)])
+def test_plain_spmv_bitwise_oracle(typ, size, k):
+    say("spmv_k", typ, size, k)
+    rp, ci, va = M.gen2(typ, size, -1.0)
+    if typ == "m9p":
+        va = va * (1.0 + 0.1 * np.random.default_rng(3).standard_normal(len(va)))
+    n = len(rp) - 1
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    A = M.CsrMatrix(rp, ci, va)
+    for graph in (False, True):
+        vv = zeros_dev(n * (k + 1))
+        vv[:n] = to_dev(x)
+        A.spmv_k(vv, k, use_graph=graph)
+        assert np.array_equal(to_host(vv), gold)
+
+
+@pytest.mark.parametrize("s", [1, 2, 3, 4])
+def test_fused_dict_bitwise_oracle(s):
+    say("fused dict s", s)
+    rp, ci, va = M.gen2("cube", 32, -1.0)
+    n = len(rp) - 1
+    part = M.partition_grid([32, 32, 32], [4, 4, 4])
+    k = 8
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    F = M.FusedMPK(rp, ci, va, part, k, s)
+    assert F.stats()["codec"] == "dict"
+    for graph in (False, True):
+        vv = zeros_dev(n * (k + 1))
+        F.exec(to_dev(x), vv, use_graph=graph)
+        out = to_host(vv)
+        assert np.array_equal(out, gold), per_power_rel_err(out, gold, n, k)
+
+
+@pytest.mark.parametrize("typ,size,blocks,k,s", [
+    ("m9p", 96, (6, 6), 16, 3), ("m5p", 64, (4, 4), 5, 2), ("m7p", 80, (5, 4), 7, 4), ("t5p", 48, (3, 3), 6, 6)])
+def test_fused_explicit_values_bitwise_oracle(typ, size, blocks, k, s):
+    say("fused explicit", typ, size, k, s)
+    rp, ci, va = M.gen2(typ, size, -1.0)
+    rng = np.random.default_rng(7)
+    va = va * (1 + 0.1 * rng.standard_normal(len(va)))
+    n = len(rp) - 1
+    part = M.partition_grid([size, size], list(blocks))
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    F = M.FusedMPK(rp, ci, va, part, k, s)
+    assert F.stats()["codec"] == "explicit"
+    vv = zeros_dev(n * (k + 1))
+    F.exec(to_dev(x), vv)
+    assert np.array_equal(to_host(vv), gold)
+
+
+def test_fused_graph_partition_bitwise_oracle():
+    say("fused graph partition")
+    rp, ci, va = M.gen2("cube", 24, -1.0)
+    n = len(rp) - 1
+    part = M.partition_graph(rp, ci, 27)
+    x = M.uniform_pm1(11, n)
+    gold = O.spmv_k(rp, ci, va, x, 6)
+    F = M.FusedMPK(rp, ci, va, part, 6, 2)
+    vv = zeros_dev(n * 7)
+    F.exec(to_dev(x), vv)
+    assert np.array_equal(to_host(vv), gold)
+
+
+def test_headline_size_128_cube_bitwise_oracle():
+    """3D 7-pt 128^3, diag +6 / off -1, x ~ U[-1,1) (seed 0x5EED), k = 8."""
+    say("headline 128^3")
+    rp, ci, va = M.gen2("cube", 128, -1.0)
+    n = len(rp) - 1
+    k = 8
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    A = M.CsrMatrix(rp, ci, va)
+    vv = zeros_dev(n * (k + 1))
+    vv[:n] = to_dev(x)
+    A.spmv_k(vv, k)
+    assert np.array_equal(to_host(vv), gold)
+    part = M.partition_grid([128, 128, 128], [8, 8, 8])
+    F = M.FusedMPK(rp, ci, va, part, k, 3)
+    vv2 = zeros_dev(n * (k + 1))
+    F.exec(to_dev(x), vv2)
+    out = to_host(vv2)
+    err = per_power_rel_err(out, gold, n, k)
+    say("per-power normwise error", err)
+    assert np.array_equal(out, gold)
+
+
+def test_exact_plan_cube64_against_oracle():
+    say("exact plan cube64")
+    rp, ci, va = M.gen2("cube", 64, -1.0)
+    n = len(rp) - 1
+    k = 8
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    for nphase in (0, 1):
+        parts = [M.partition_grid([64] * 3, [2, 2, 2], [16 * ph] * 3) for ph in range(max(nphase, 1))]
+        _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
+        plan = M.Plan(bufs)
+        vv = zeros_dev(n * (k + 1))
+        plan.exec(to_dev(x), vv)
+        assert np.array_equal(to_host(vv), gold)

@@ -1,0 +1,219 @@
+"""The library's host prep (levels, skirt, LAPJV, fill_buffers, prep_mpk,
+formats) against the reference outputs in tests/golden and the oracle."""
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import mpk_amd as M
+import tests.oracle_api as O
+
+CASES = O.cases()
+
+
+def _ref_levels(case):
+    d, rp, ci, va, parts, levs = O.case_inputs(case)
+    return d, rp, ci, va, parts, levs
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_generator_and_text_formats(case):
+    d = O.digests(case)
+    rp, ci, va = M.gen2(d["type"], d["size"])
+    orp, oci, ova = O.gen2(d["type"], d["size"])
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(ci, oci)
+    np.testing.assert_array_equal(va, ova)
+    with tempfile.TemporaryDirectory() as t:
+        M.write_crs(f"{t}/loop.g0", rp, ci)
+        M.write_vals(f"{t}/loop.val", va)
+        assert O.sha(open(f"{t}/loop.g0", "rb").read()) == d["loop.g0"]
+        assert O.sha(open(f"{t}/loop.val", "rb").read()) == d["loop.val"]
+        rp2, ci2 = M.read_crs(f"{t}/loop.g0")
+        np.testing.assert_array_equal(rp2, rp)
+        np.testing.assert_array_equal(ci2, ci)
+        np.testing.assert_array_equal(M.read_vals(f"{t}/loop.val", len(va)), va)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_levels_skirt_byte_equal_reference(case, golden_dir):
+    d, rp, ci, va, parts, levs = _ref_levels(case)
+    prev = None
+    with tempfile.TemporaryDirectory() as t:
+        for ph in range(d["nphase"]):
+            lv = M.comp_level(rp, ci, parts[ph]) if prev is None else M.update_level(rp, ci, parts[ph], prev)
+            M.write_ints(f"{t}/l{ph}", lv)
+            assert open(f"{t}/l{ph}", "rb").read() == open(f"{golden_dir}/{case}/l{ph}", "rb").read()
+            prev = lv
+        sk = M.comp_skirt(rp, ci, parts[0], d["npart"], d["nlevel"], prev)
+        sk.write(f"{t}/s")
+        ref = open(f"{golden_dir}/{case}/s{d['nphase']}", "rb").read()
+        assert open(f"{t}/s", "rb").read() == ref
+        back = M.read_skirt(f"{golden_dir}/{case}/s{d['nphase']}", d["n"], d["npart"], d["nlevel"])
+        np.testing.assert_array_equal(back.dense(), sk.dense())
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fill_buffers_byte_equal_reference(case):
+    d, rp, ci, va, parts, levs = _ref_levels(case)
+    sk = M.comp_skirt(rp, ci, parts[0], d["npart"], d["nlevel"], levs[-1] if d["nphase"] else None)
+    bufs, relab = M.fill_buffers(rp, ci, va, d["nlevel"], d["nphase"], [p.copy() for p in parts], levs, sk)
+    for r in range(d["npart"]):
+        b = bufs.rank_bytes(r)
+        assert len(b) == d["bufs"][str(r)]["bytes"]
+        assert O.sha(b) == d["bufs"][str(r)]["sha256"], f"rank {r}"
+    # the oracle relabels the same way
+    _, orelab = O.fill_buffers(rp, ci, va, d["nlevel"], d["nphase"], parts, levs,
+                               O.skirt_filter(O.skirt_dist(rp, ci, parts[0]), d["nlevel"],
+                                              levs[-1] if d["nphase"] else None))
+    for a, b in zip(relab, orelab):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_read_write_buffers_round_trip(golden_dir):
+    for case in ["m5p8_4_4_1", "cube8_8_4_0"]:
+        d = O.digests(case)
+        src = os.path.join(golden_dir, case)
+        bs = M.read_buffers(src, d["npart"])
+        with tempfile.TemporaryDirectory() as t:
+            bs.write(t)
+            for r in range(d["npart"]):
+                assert open(f"{t}/rank{r}.bufs", "rb").read() == open(f"{src}/rank{r}.bufs", "rb").read()
+        assert bs[0].nphase == d["nphase"] and bs[0].npart == d["npart"]
+        bs.free()
+
+
+def test_read_buffers_errors():
+    with pytest.raises(M.MpkxError):
+        M.read_buffers("/nonexistent/dir", 1)
+    with tempfile.TemporaryDirectory() as t:
+        open(f"{t}/rank0.bufs", "wb").write(b"\x01\x00")
+        with pytest.raises(M.MpkxError):
+            M.read_buffers(t, 1)
+
+
+def test_lapjv_same_permutation_as_oracle():
+    rng = np.random.default_rng(5)
+    for size in (2, 3, 5, 8, 17, 40):
+        for _ in range(200 if size < 10 else 20):
+            s = rng.integers(0, 6, (size, size)).astype(np.int32)  # many ties
+            np.testing.assert_array_equal(M.lapjv(s), O.lapjv(s))
+
+
+def test_levels_random_partitions_match_oracle():
+    rng = np.random.default_rng(1)
+    for typ, size in (("m5p", 13), ("m9p", 11), ("t5p", 9), ("cube", 6)):
+        rp, ci, va = M.gen2(typ, size)
+        n = len(rp) - 1
+        for npart in (2, 3, 7):
+            part = rng.integers(0, npart, n).astype(np.int32)
+            part[:npart] = np.arange(npart)
+            l0 = M.comp_level(rp, ci, part)
+            np.testing.assert_array_equal(l0, O.levels(rp, ci, part))
+            part2 = np.roll(part, 5)
+            l1 = M.update_level(rp, ci, part2, l0)
+            np.testing.assert_array_equal(l1, O.levels(rp, ci, part2, l0))
+            for nl in (2, 4, 9):
+                sk = M.comp_skirt(rp, ci, part, npart, nl, None)
+                ref = O.skirt_filter(O.skirt_dist(rp, ci, part), nl, None)
+                np.testing.assert_array_equal(sk.dense(), ref)
+                sk1 = M.comp_skirt(rp, ci, part, npart, nl, l1)
+                ref1 = O.skirt_filter(O.skirt_dist(rp, ci, part), nl, l1)
+                np.testing.assert_array_equal(sk1.dense(), ref1)
+
+
+def test_fill_buffers_graph_partitions_match_oracle():
+    """Irregular partitions (graph partitioner, random relabel) and 3 phases."""
+    rng = np.random.default_rng(2)
+    rp, ci, va = M.gen2("m9p", 14)
+    va = va * (1 + 0.1 * rng.standard_normal(len(va)))
+    npart, nlevel, nphase = 5, 5, 3
+    parts = [M.partition_graph(rp, ci, npart)]
+    for ph in range(1, nphase):
+        parts.append(rng.permutation(npart).astype(np.int32)[np.roll(parts[0], 17 * ph)])
+    levels, sk, bufs, relab = M.prepare(rp, ci, va, nlevel, nphase, parts)
+    ol = []
+    prev = None
+    for ph in range(nphase):
+        prev = O.levels(rp, ci, parts[ph], prev)
+        ol.append(prev)
+        np.testing.assert_array_equal(prev, levels[ph])
+    sl = O.skirt_filter(O.skirt_dist(rp, ci, parts[0]), nlevel, ol[-1])
+    blobs, _ = O.fill_buffers(rp, ci, va, nlevel, nphase, parts, ol, sl)
+    for r in range(npart):
+        assert bufs.rank_bytes(r) == blobs[r]
+
+
+def test_prep_mpk_task_lists_match_oracle():
+    for case in ["m5p8_4_4_0", "m5p64_4_4_2", "cube16_8_8_1", "m9p16_8_6_2"]:
+        d, rp, ci, va, parts, levs = _ref_levels(case)
+        sk = M.comp_skirt(rp, ci, parts[0], d["npart"], d["nlevel"], levs[-1] if d["nphase"] else None)
+        tcount, tidx = M.prep_mpk(rp, ci, d["nlevel"], d["nphase"], parts, levs, sk)
+        sl = O.skirt_filter(O.skirt_dist(rp, ci, parts[0]), d["nlevel"], levs[-1] if d["nphase"] else None)
+        tot, otc, oti = O.prep_tasks(rp, ci, d["nlevel"], d["nphase"], parts, levs, sl)
+        np.testing.assert_array_equal(tcount, otc)
+        np.testing.assert_array_equal(tidx, oti)
+
+
+def test_prep_mpk_structure_error():
+    d, rp, ci, va, parts, levs = _ref_levels("m5p8_4_4_1")
+    bad = levs[0].copy()
+    bad[10] += 3  # a level that its neighbours cannot support
+    sk = M.comp_skirt(rp, ci, parts[0], d["npart"], d["nlevel"], bad)
+    with pytest.raises(M.MpkxError) as e:
+        M.prep_mpk(rp, ci, d["nlevel"], d["nphase"], parts, [bad], sk)
+    assert e.value.code == -5
+
+
+def test_partition_grid_matches_fixture_partitions(golden_dir):
+    # the fixture partitions were written by make_golden.block_partition
+    from tests.golden.make_golden import block_partition
+    for size, ndim, blocks in ((64, 2, (2, 2)), (16, 3, (2, 2, 2)), (12, 2, (3, 4))):
+        for shift in (0, 5):
+            ours = M.partition_grid([size] * ndim, blocks, [shift] * ndim)
+            np.testing.assert_array_equal(ours, block_partition(size, ndim, blocks, shift))
+
+
+def test_partition_graph_balanced_connected():
+    rp, ci, va = M.gen2("cube", 12)
+    n = len(rp) - 1
+    for npart in (2, 5, 16):
+        part = M.partition_graph(rp, ci, npart)
+        sizes = np.bincount(part, minlength=npart)
+        assert sizes.min() > 0 and sizes.max() - sizes.min() <= max(2, n // npart // 4)
+
+
+def test_mtx_reader(tmp_path):
+    p = tmp_path / "a.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real symmetric\n% c\n3 3 4\n1 1 2.0\n2 1 -1\n3 2 -1\n3 3 2\n")
+    rp, ci, va = M.read_mtx(str(p))
+    np.testing.assert_array_equal(rp, [0, 2, 4, 6])
+    np.testing.assert_array_equal(ci, [0, 1, 0, 2, 1, 2])
+    np.testing.assert_array_equal(va, [2, -1, -1, -1, -1, 2])
+    q = tmp_path / "b.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate pattern general\n2 2 3\n1 1\n1 2\n2 2\n")
+    rp, ci, va = M.read_mtx(str(q))
+    np.testing.assert_array_equal(va, [0.5, 0.5, 1.0])
+    r = tmp_path / "c.mtx"
+    r.write_text("%%MatrixMarket matrix array real general\n2 2\n")
+    with pytest.raises(M.MpkxError):
+        M.read_mtx(str(r))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    """Every function declared in include/*.h is exported by libmpkx.so."""
+    import ctypes
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    names = set()
+    for h in os.listdir(os.path.join(repo, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(repo, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(mpkx_[a-z0-9_]+)\s*\(", txt))
+    assert len(names) > 30
+    L = ctypes.CDLL(M.LIBPATH)
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert M.lib().mpkx_version().startswith(b"mpkx")
