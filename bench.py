@@ -30,7 +30,7 @@ def algorithmic_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 8 * n + 8 * n
 
 
-def cpu_baseline(rp, ci, va, x, k, budget_s):
+def cpu_baseline_port(rp, ci, va, x, k, budget_s):
     """Oracle k x SpMV (the CPU restatement of verify_val), one core, on the
     full workload, repeated for about `budget_s` seconds."""
     from tests import oracle_api as O
@@ -48,8 +48,65 @@ def cpu_baseline(rp, ci, va, x, k, budget_s):
             break
     B = algorithmic_bytes(n, len(ci))
     return {"value": round(k * B / best / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle k x CSR SpMV (mpk2/verify_val.c order) on the full 128^3 k=8 "
+            "sample": f"oracle k x CSR SpMV (mpk2/verify_val.c order) on the full {round(n ** (1/3))}^3 k={k} "
                       f"workload, min of {reps} reps ({time.perf_counter() - t0:.1f} s)"}
+
+
+def cpu_baseline_reference(M, rp, ci, va, x, N, k, gold_check):
+    """The reference MPI exec (mpk2/mpi2_mpkexecbufs_val.c, built from the
+    reference sources into oracle/_ref/ref_exec) on the host cores: one rank per
+    partition, P = the largest power of two <= min(cores, 16), block partitions,
+    nphase = 0, rank%d.bufs written by this library's prep (byte-identical to
+    the reference prep where that one fits in memory).  The reference's own
+    timing protocol: min of 5 repetitions bracketed by MPI_Barrier."""
+    import shutil
+    import subprocess
+    import tempfile
+    ref = os.path.join(REPO, "oracle", "_ref", "ref_exec")
+    mpirun = "/opt/conda/bin/mpirun"
+    if not (os.path.exists(ref) and os.path.exists(mpirun)):
+        return None
+    cores = min(os.cpu_count() or 1, 16)
+    P = 1
+    while P * 2 <= cores:
+        P *= 2
+    blocks = [1, 1, 1]
+    d = 0
+    q = P
+    while q > 1:
+        blocks[d % 3] *= 2
+        q //= 2
+        d += 1
+    part = M.partition_grid([N, N, N], blocks)
+    sk = M.comp_skirt(rp, ci, part, P, k, None)
+    bufs, _ = M.fill_buffers(rp, ci, va, k, 0, [part], [], sk)
+    tmp = tempfile.mkdtemp(prefix="mpkx_ref_")
+    try:
+        run = os.path.join(tmp, f"cube{N}_{P}_{k}_0")
+        os.makedirs(run)
+        bufs.write(run)
+        bufs.free()
+        x.tofile(os.path.join(tmp, "x.bin"))
+        t0 = time.perf_counter()
+        r = subprocess.run([mpirun, "-launcher", "fork", "-n", str(P), ref, run,
+                            os.path.join(tmp, "x.bin"), "5"], capture_output=True, text=True,
+                           timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+        line = [l for l in r.stdout.splitlines() if l.startswith("ref_exec")][-1]
+        t = float(line.split("min_seconds=")[1].split()[0])
+        res = np.fromfile(os.path.join(run, "results.bin"))
+        ok = bool(gold_check(res))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    n = len(rp) - 1
+    B = algorithmic_bytes(n, len(ci))
+    return {"value": round(k * B / t / 1e9, 3), "unit": "GB/s", "cores": P, "kind": "reference",
+            "sample": f"reference mpi2_mpkexecbufs_val exec (MPICH, {P} ranks, block partitions "
+                      f"{blocks}, nphase 0) on the full {N}^3 k={k} workload, min of 5 reps, "
+                      f"{t * 1e3:.2f} ms per MPK; {wall:.1f} s wall incl. loading buffers",
+            "results_bitwise_equal_gpu": ok}
 
 
 def main():
@@ -163,7 +220,15 @@ def main():
                      "round_ms": [round(t, 5) for t in round_ms]},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(rp, ci, va, x, k, args.cpu_seconds)
+        host_vv = vf.cpu().numpy()
+        cb = None
+        try:
+            cb = cpu_baseline_reference(M, rp, ci, va, x, N, k, lambda r: np.array_equal(r, host_vv))
+        except Exception as e:  # fall back to the oracle port
+            print(f"reference CPU baseline unavailable: {e}", file=sys.stderr)
+        if cb is None:
+            cb = cpu_baseline_port(rp, ci, va, x, k, args.cpu_seconds)
+        out["cpu_baseline"] = cb
     print(json.dumps(out))
 
 
