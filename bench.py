@@ -109,6 +109,29 @@ def cpu_baseline_reference(M, rp, ci, va, x, N, k, gold_check):
             "results_bitwise_equal_gpu": ok}
 
 
+def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
+    """k back-to-back SpMVs through torch.sparse CSR (hipSPARSE/rocSPARSE on
+    ROCm), as an outside reference point for the plain kernel."""
+    n = len(rp) - 1
+    A = torch.sparse_csr_tensor(torch.from_numpy(rp.astype(np.int64)), torch.from_numpy(ci.astype(np.int64)),
+                                torch.from_numpy(va), size=(n, n)).to("cuda:0")
+    v = xd.clone().unsqueeze(1)
+    for _ in range(2):
+        w = v
+        for _ in range(k):
+            w = A @ w
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        w = v
+        for _ in range(k):
+            w = A @ w
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +141,9 @@ def main():
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--s", type=int, default=3, help="levels per fused launch")
     ap.add_argument("--blocks", type=int, default=8, help="tiles per dimension")
+    ap.add_argument("--s32", type=int, default=4, help="levels per launch, FP32 form")
+    ap.add_argument("--blocks32", type=int, default=8, help="tiles per dimension, FP32 form")
+    ap.add_argument("--cold-reps", type=int, default=10, help="cold-MALL repetitions (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -182,6 +208,58 @@ def main():
     torch.cuda.synchronize()
     parity = parity and bool(torch.equal(vf, vp))
 
+    # cold MALL: a 512 MiB write between repetitions evicts the 256 MB
+    # Infinity Cache; each repetition timed alone with events
+    def cold(fn, reps):
+        scratch = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        times = []
+        for _ in range(reps):
+            scratch.fill_(1.0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        del scratch
+        return float(np.median(times))
+
+    cold_ms = None
+    if args.cold_reps > 0:
+        cold_ms = {"fused": cold(lambda: F.exec(xd, vf), args.cold_reps),
+                   "kxspmv": cold(lambda: A.spmv_k(vp, k), args.cold_reps)}
+
+    # reduced-precision form: FP32 vectors/accumulation, exact matrix values
+    # (dictionary), per-power scaling by 1/sigma
+    fp32 = None
+    try:
+        part32 = M.partition_grid([N, N, N], [args.blocks32] * 3)
+        F32 = M.FusedMPK32(rp, ci, va, part32, k, args.s32)
+        v32 = torch.zeros(n * (k + 1), dtype=torch.float32, device=dev)
+        for _ in range(args.warmup):
+            F32.exec(xd, v32)
+        ms32, _ = timed(lambda: F32.exec(xd, v32))
+        ref = vf.view(k + 1, n)
+        got = v32.view(k + 1, n).double()
+        scale = torch.tensor([F32.sigma ** l for l in range(k + 1)], dtype=torch.float64, device=dev)
+        errs = (torch.linalg.norm(got * scale[:, None] - ref, dim=1) / torch.linalg.norm(ref, dim=1)).cpu().numpy()
+        fp32 = {"ms_per_step": round(ms32, 5), "value": round(k * algorithmic_bytes(n, nnz) / (ms32 * 1e-3) / 1e9, 2),
+                "unit": "GB/s (FP64-equivalent bytes)", "speedup_vs_kxspmv_fp64": round(ms_plain / ms32, 3),
+                "per_power_rel_err_vs_fp64": [float("%.3e" % e) for e in errs],
+                "levels_per_launch": args.s32, "tiles": args.blocks32 ** 3, "plan": F32.stats()}
+        if cold_ms is not None:
+            cold_ms["fused_fp32"] = cold(lambda: F32.exec(xd, v32), args.cold_reps)
+        del v32, F32
+    except Exception as e:
+        fp32 = {"error": str(e)}
+
+    try:
+        ms_rs = rocsparse_kxspmv(torch, rp, ci, va, xd, k, max(3, args.steps // 3))
+    except Exception as e:
+        ms_rs = None
+        print(f"rocSPARSE reference point unavailable: {e}", file=sys.stderr)
+
     B = algorithmic_bytes(n, nnz)
     gbs_fused = k * B / (ms_fused * 1e-3) / 1e9
     gbs_plain = k * B / (ms_plain * 1e-3) / 1e9
@@ -211,6 +289,10 @@ def main():
         "baseline_kxspmv": {"value": round(gbs_plain, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 5),
                             "pct_hbm_roofline": round(100 * gbs_plain / HBM_PEAK_GBS, 2)},
         "speedup_vs_kxspmv": round(ms_plain / ms_fused, 3),
+        "kxspmv_rocsparse": None if ms_rs is None else {
+            "ms_per_step": round(ms_rs, 5), "value": round(k * B / (ms_rs * 1e-3) / 1e9, 2), "unit": "GB/s"},
+        "cold_mall_ms_per_step": None if cold_ms is None else {kk: round(v, 5) for kk, v in cold_ms.items()},
+        "fp32_form": fp32,
         "wall_ms_per_step": {"fused": round(wall_fused, 5), "kxspmv": round(wall_plain, 5)},
         "parity_bitwise_fused_vs_kxspmv": parity,
         "fused_plan": st,

@@ -253,6 +253,37 @@ int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots,
                      long *row_levels, long *matrix_bytes, int *codec,
                      int *launches);
 
+/* ------------------------------------------------------------------ */
+/* Reduced-precision fused MPK (no reference counterpart: the reference */
+/* is FP64 only).  FP32 vectors and FP32 accumulation; matrix values as */
+/* a <= 4-value dictionary (MPKX_VALUES_AUTO, when the matrix has that  */
+/* few distinct values), an FP32 stream or a BF16 stream.  Every power  */
+/* is renormalised by 1/sigma, sigma = the power of two at or above the */
+/* largest row abs-sum, so the scaling adds no rounding:                */
+/*   vv[l*n + i] = (A^l x)_i / sigma^l.                                 */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_fused32 mpkx_fused32;
+enum { MPKX_VALUES_AUTO = 0, MPKX_VALUES_FP32 = 1, MPKX_VALUES_BF16 = 2 };
+int mpkx_fused32_create(mpkx_fused32 **f, int n, const int *ptr, const int *col,
+                        const double *val, const int *part, int npart,
+                        int nlevel, int s, int values, double *sigma);
+void mpkx_fused32_destroy(mpkx_fused32 *f);
+/* x: device FP64 (n); vv: device FP32 (n*(nlevel+1)). */
+int mpkx_fused32_exec(mpkx_fused32 *f, const double *x, float *vv,
+                      void *stream, int use_graph);
+int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream);
+/* values: 0 dictionary, 1 FP32 stream, 2 BF16 stream. */
+int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots,
+                       long *row_levels, long *matrix_bytes, int *values,
+                       int *launches);
+/* FP64 checks on the GPU.  norms[l] = ||vv[l*n ..]||_2 for l = 0..nlevel
+ * (host array of nlevel+1).  residual = ||sigma*v_l - A v_{l-1}||_2 /
+ * ||A v_{l-1}||_2 with A applied in FP64 (the mpkx_csr matrix). */
+int mpkx_norms32(const float *vv, int n, int nlevel, double *norms,
+                 void *stream);
+int mpkx_residual32(mpkx_csr *A, const float *vv, int level, double sigma,
+                    double *residual, void *stream);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

@@ -53,6 +53,14 @@ def lib():
         return _lib
     if not os.path.exists(LIBPATH):
         raise RuntimeError(f"{LIBPATH} is missing: run __graft_entry__.build() first")
+    # torch ships its own libamdhip64.so / libhsa-runtime64.so with the same
+    # SONAMEs as /opt/rocm's.  Loading torch first makes libmpkx bind to that
+    # already-loaded runtime, so one process never holds two HIP runtimes
+    # (the second one cannot see the GPU).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     sig = {
         "mpkx_last_error": ([], C.c_char_p),
@@ -95,6 +103,13 @@ def lib():
         "mpkx_fused_exec": ([P, P, P, P, C.c_int], C.c_int),
         "mpkx_fused_stats": ([P, P, P, P, P, P, P], C.c_int),
         "mpkx_fused_round": ([P, P, C.c_int, P], C.c_int),
+        "mpkx_fused32_create": ([P, C.c_int, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, P], C.c_int),
+        "mpkx_fused32_destroy": ([P], None),
+        "mpkx_fused32_exec": ([P, P, P, P, C.c_int], C.c_int),
+        "mpkx_fused32_round": ([P, P, C.c_int, P], C.c_int),
+        "mpkx_fused32_stats": ([P, P, P, P, P, P, P], C.c_int),
+        "mpkx_norms32": ([P, C.c_int, C.c_int, P, P], C.c_int),
+        "mpkx_residual32": ([P, P, C.c_int, C.c_double, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -505,3 +520,58 @@ def prepare(ptr, col, val, nlevel, nphase, parts):
     sk = comp_skirt(ptr, col, parts[0], npart, nlevel, levels[-1] if nphase else None)
     bufs, parts = fill_buffers(ptr, col, val, nlevel, nphase, [p.copy() for p in parts], levels, sk)
     return levels, sk, bufs, parts
+
+
+class FusedMPK32:
+    """Reduced-precision fused MPK: FP32 vectors/accumulation, matrix values as
+    a dictionary (AUTO), FP32 or BF16; vv[l] = A^l x / sigma^l (exact power-of-two
+    scaling per power)."""
+
+    AUTO, FP32, BF16 = 0, 1, 2
+
+    def __init__(self, ptr, col, val, part, nlevel, s, values=0):
+        self._h = C.c_void_p()
+        part = _i32(part)
+        self.npart = int(part.max()) + 1
+        self.n = len(ptr) - 1
+        self.nlevel = nlevel
+        self.s = s
+        sig = C.c_double()
+        _check(lib().mpkx_fused32_create(C.byref(self._h), self.n, _p(_i32(ptr)), _p(_i32(col)),
+                                         _p(np.ascontiguousarray(val, np.float64)), _p(part),
+                                         self.npart, nlevel, s, values, C.byref(sig)))
+        self.sigma = sig.value
+
+    def exec(self, x, vv, stream=None, use_graph=True):
+        _check(lib().mpkx_fused32_exec(self._h, _dev(x), _dev(vv), _stream(stream), int(use_graph)))
+
+    def round(self, vv, r, stream=None):
+        _check(lib().mpkx_fused32_round(self._h, _dev(vv), r, _stream(stream)))
+
+    def stats(self):
+        ms, sl, rl, mb = C.c_long(), C.c_long(), C.c_long(), C.c_long()
+        vals, la = C.c_int(), C.c_int()
+        _check(lib().mpkx_fused32_stats(self._h, C.byref(ms), C.byref(sl), C.byref(rl), C.byref(mb),
+                                        C.byref(vals), C.byref(la)))
+        return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
+                "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16"][vals.value],
+                "launches": la.value, "sigma": self.sigma}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_fused32_destroy(self._h)
+            self._h = None
+
+
+def norms32(vv, n, nlevel, stream=None):
+    """FP64 2-norms (computed on the GPU) of every power of an FP32 Krylov block."""
+    out = np.zeros(nlevel + 1)
+    _check(lib().mpkx_norms32(_dev(vv), n, nlevel, _p(out), _stream(stream)))
+    return out
+
+
+def residual32(csr, vv, level, sigma, stream=None):
+    """FP64 check ||sigma*v_l - A v_{l-1}|| / ||A v_{l-1}|| of an FP32 Krylov block."""
+    r = C.c_double()
+    _check(lib().mpkx_residual32(csr._h, _dev(vv), level, sigma, C.byref(r), _stream(stream)))
+    return r.value

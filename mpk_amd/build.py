@@ -21,6 +21,9 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             # packed FP32 instructions are kept out of every device code object
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-fno-gpu-flush-denormals-to-zero", "-Wno-unused-command-line-argument",
+            # FP64 kernels must round every product and sum separately (the
+            # reference's operation order); FP32 code calls fmaf explicitly
+            "-ffp-contract=off",
             "-fvisibility=hidden"]
 
 

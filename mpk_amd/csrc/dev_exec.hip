@@ -16,6 +16,7 @@
 // as the reference's `tmp += mval[mj] * vv_buf[mcol[mj]]`, so results are
 // bitwise equal to the reference and to verify_val.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <unordered_set>
@@ -435,6 +436,114 @@ int mpkx_spmv_k(mpkx_csr *m, double *vv, int k, void *stream, int use_graph) {
     m->graph.k = k;
   }
   MPKX_HIP(hipGraphLaunch(m->graph.exec, st));
+  return MPKX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FP64 checks of the reduced-precision form
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+__device__ double block_sum(double v) {
+  __shared__ double red[RED_THREADS];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_sumsq32(const float *__restrict__ v, long n,
+                                                         double *__restrict__ partial) {
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (long)gridDim.x * RED_THREADS) {
+    double d = v[i];
+    acc += d * d;
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// partial[2*b] = sum (sigma*v_l - A v_{l-1})^2, partial[2*b+1] = sum (A v_{l-1})^2
+__global__ __launch_bounds__(RED_THREADS) void k_resid32(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                         const double *__restrict__ va,
+                                                         const float *__restrict__ prev,
+                                                         const float *__restrict__ cur, double sigma, int n,
+                                                         double *__restrict__ partial) {
+  double a2 = 0.0, r2 = 0.0;
+  for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += gridDim.x * RED_THREADS) {
+    double acc = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; j++) acc += va[j] * (double)prev[ci[j]];
+    double d = sigma * (double)cur[i] - acc;
+    r2 += d * d;
+    a2 += acc * acc;
+  }
+  double s1 = block_sum(r2);
+  double s2 = block_sum(a2);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s1;
+    partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_finish(const double *__restrict__ partial, int count,
+                                                        int stride, double *__restrict__ out) {
+  for (int c = 0; c < stride; c++) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < count; i += RED_THREADS) acc += partial[(long)i * stride + c];
+    double s = block_sum(acc);
+    if (threadIdx.x == 0) out[c] = s;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpkx_norms32(const float *vv, int n, int nlevel, double *norms, void *stream) {
+  clear_error();
+  if (!vv || n <= 0 || nlevel < 0 || !norms) return fail(MPKX_EINVAL, "norms32: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DevArray<double> partial, out;
+  int rc;
+  if ((rc = partial.alloc((size_t)RED_BLOCKS)) || (rc = out.alloc((size_t)nlevel + 1))) return rc;
+  for (int l = 0; l <= nlevel; l++) {
+    hipLaunchKernelGGL(k_sumsq32, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, vv + (size_t)l * n, (long)n,
+                       partial.p);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(RED_THREADS), 0, st, partial.p, RED_BLOCKS, 1, out.p + l);
+  }
+  MPKX_HIP(hipGetLastError());
+  std::vector<double> h(nlevel + 1);
+  MPKX_HIP(hipMemcpyAsync(h.data(), out.p, sizeof(double) * (nlevel + 1), hipMemcpyDeviceToHost, st));
+  MPKX_HIP(hipStreamSynchronize(st));
+  for (int l = 0; l <= nlevel; l++) norms[l] = std::sqrt(h[l]);
+  return MPKX_OK;
+}
+
+int mpkx_residual32(mpkx_csr *A, const float *vv, int level, double sigma, double *residual, void *stream) {
+  clear_error();
+  if (!A || !vv || level < 1 || !residual) return fail(MPKX_EINVAL, "residual32: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DevArray<double> partial, out;
+  int rc;
+  if ((rc = partial.alloc((size_t)2 * RED_BLOCKS)) || (rc = out.alloc(2))) return rc;
+  const float *prev = vv + (size_t)(level - 1) * A->n, *cur = vv + (size_t)level * A->n;
+  hipLaunchKernelGGL(k_resid32, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, A->rp.p, A->ci.p, A->va.p, prev,
+                     cur, sigma, A->n, partial.p);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(RED_THREADS), 0, st, partial.p, RED_BLOCKS, 2, out.p);
+  MPKX_HIP(hipGetLastError());
+  double h[2];
+  MPKX_HIP(hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, st));
+  MPKX_HIP(hipStreamSynchronize(st));
+  *residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
   return MPKX_OK;
 }
 

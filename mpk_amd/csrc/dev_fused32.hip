@@ -1,0 +1,188 @@
+// dev_fused32.hip -- fused matrix powers kernel, reduced-precision form.
+//
+// Vectors and accumulation in FP32; matrix values as a <= 4-value
+// dictionary, an FP32 stream or a BF16 stream.  Every new power is scaled by
+// 1/sigma with sigma the power of two at or above max_i sum_j |a_ij|, so the
+// scaling itself is exact and the monomial basis stays in range:
+//   vv[l*n + i] = (A^l x)_i / sigma^l   (up to FP32 rounding).
+// FP64 appears only in the norms and the residual check (dev_exec.hip).
+#include <cmath>
+
+#include "fused_impl.h"
+
+using namespace mpkx;
+
+namespace {
+
+__global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (float)src[t];
+}
+
+using K32f = FusedKernel<float, float>;
+using K32b = FusedKernel<float, uint16_t>;
+
+template <typename MT>
+FusedKernel<float, MT> pick32(int T, int MR, int W, bool dict) {
+  if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4>(W, dict);
+  if (T == 1024 && MR == 4) return pick_fused_w<float, MT, 1024, 4>(W, dict);
+  if (T == 1024 && MR == 8) return pick_fused_w<float, MT, 1024, 8>(W, dict);
+  if (T == 1024 && MR == 16) return pick_fused_w<float, MT, 1024, 16>(W, dict);
+  return nullptr;
+}
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+uint16_t to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+}  // namespace
+
+struct mpkx_fused32 {
+  TilePlan plan;
+  int values = 0;  // 0 dictionary, 1 FP32 stream, 2 BF16 stream
+  float dv[4] = {0, 0, 0, 0};
+  float inv_sigma = 1.0f;
+  double sigma = 1.0;
+  size_t lds_bytes = 0;
+  long slots = 0, matrix_bytes = 0;
+  K32f fnf = nullptr;
+  K32b fnb = nullptr;
+  DevArray<int> hdr, grow;
+  DevArray<uint32_t> ent;
+  DevArray<float> valf;
+  DevArray<uint16_t> valb;
+  GraphCache graph;
+};
+
+static void launch_round32(mpkx_fused32 *f, float *vv, int L0, hipStream_t st) {
+  const TilePlan &P = f->plan;
+  int w = std::min(P.s, P.nlevel - L0);
+  if (f->values == 2)
+    hipLaunchKernelGGL(f->fnb, dim3(P.ntiles), dim3(P.T), f->lds_bytes, st, f->hdr.p, f->grow.p, f->ent.p,
+                       f->valb.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], f->inv_sigma, vv, P.n, L0, w,
+                       P.ntiles, P.maxslots);
+  else
+    hipLaunchKernelGGL(f->fnf, dim3(P.ntiles), dim3(P.T), f->lds_bytes, st, f->hdr.p, f->grow.p, f->ent.p,
+                       f->valf.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], f->inv_sigma, vv, P.n, L0, w,
+                       P.ntiles, P.maxslots);
+}
+
+static void fused32_launches(mpkx_fused32 *f, const double *x, float *vv, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((f->plan.n + 255) / 256)), dim3(256), 0, st, vv, x,
+                     (long)f->plan.n);
+  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s) launch_round32(f, vv, L0, st);
+}
+
+extern "C" {
+
+int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *col, const double *val,
+                        const int *part, int npart, int nlevel, int s, int values, double *sigma_out) {
+  clear_error();
+  if (!out || values < 0 || values > 2) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
+  auto *f = new (std::nothrow) mpkx_fused32;
+  if (!f) return fail(MPKX_ENOMEM, "fused32_create");
+  std::unique_ptr<mpkx_fused32> guard(f);
+  int rc;
+  try {
+    rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, values == MPKX_VALUES_AUTO,
+                         2 * (int)sizeof(float), {{512, 4}, {1024, 4}, {1024, 8}, {1024, 16}}, f->plan);
+  } catch (const std::bad_alloc &) {
+    return fail(MPKX_ENOMEM, "fused32_create: host allocation");
+  }
+  if (rc) return rc;
+  TilePlan &P = f->plan;
+  f->values = P.dict ? 0 : (values == MPKX_VALUES_BF16 ? 2 : 1);
+  for (size_t i = 0; i < P.dictv.size(); i++) f->dv[i] = (float)P.dictv[i];
+  // sigma: power of two >= the largest row abs-sum (a bound on ||A||_inf)
+  double mx = 0.0;
+  for (int i = 0; i < n; i++) {
+    double r = 0.0;
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) r += std::fabs(val[j]);
+    mx = std::max(mx, r);
+  }
+  int e2 = 0;
+  if (mx > 0) {
+    double m = std::frexp(mx, &e2);  // mx = m * 2^e2, m in [0.5, 1)
+    if (m == 0.5) e2 -= 1;           // exact power of two
+  }
+  f->sigma = std::ldexp(1.0, e2);
+  f->inv_sigma = (float)(1.0 / f->sigma);
+  if (sigma_out) *sigma_out = f->sigma;
+  if (f->values == 2) {
+    f->fnb = pick32<uint16_t>(P.T, P.MR, P.W, false);
+    if (!f->fnb) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
+  } else {
+    f->fnf = pick32<float>(P.T, P.MR, P.W, P.dict);
+    if (!f->fnf) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
+  }
+  f->lds_bytes = (size_t)2 * P.maxslots * sizeof(float);
+  f->slots = (long)P.grow.size();
+  std::vector<float> vf;
+  std::vector<uint16_t> vb;
+  if (f->values == 1) {
+    vf.resize(P.vals.size());
+    for (size_t i = 0; i < vf.size(); i++) vf[i] = (float)P.vals[i];
+  } else if (f->values == 2) {
+    vb.resize(P.vals.size());
+    for (size_t i = 0; i < vb.size(); i++) vb[i] = to_bf16((float)P.vals[i]);
+  }
+  f->matrix_bytes = (long)P.ent.size() * 4 + (long)vf.size() * 4 + (long)vb.size() * 2;
+  if (vf.empty()) vf.push_back(0.0f);
+  if (vb.empty()) vb.push_back(0);
+  if ((rc = f->hdr.upload(P.hdr)) || (rc = f->grow.upload(P.grow)) || (rc = f->ent.upload(P.ent)) ||
+      (rc = f->valf.upload(vf)) || (rc = f->valb.upload(vb)))
+    return rc;
+  P.grow.clear();
+  P.grow.shrink_to_fit();
+  P.ent.clear();
+  P.ent.shrink_to_fit();
+  P.vals.clear();
+  P.vals.shrink_to_fit();
+  const void *fn = f->values == 2 ? (const void *)f->fnb : (const void *)f->fnf;
+  MPKX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)f->lds_bytes));
+  *out = guard.release();
+  return MPKX_OK;
+}
+
+void mpkx_fused32_destroy(mpkx_fused32 *f) { delete f; }
+
+int mpkx_fused32_exec(mpkx_fused32 *f, const double *x, float *vv, void *stream, int use_graph) {
+  clear_error();
+  if (!f || !x || !vv) return fail(MPKX_EINVAL, "fused32_exec: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    fused32_launches(f, x, vv, st);
+    MPKX_HIP(hipGetLastError());
+    return MPKX_OK;
+  }
+  return graph_run(f->graph, x, vv, st, [&](hipStream_t s) { fused32_launches(f, x, vv, s); });
+}
+
+int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream) {
+  clear_error();
+  if (!f || !vv || round < 0 || (long)round * f->plan.s >= f->plan.nlevel)
+    return fail(MPKX_EINVAL, "fused32_round: bad arguments");
+  launch_round32(f, vv, round * f->plan.s, (hipStream_t)stream);
+  MPKX_HIP(hipGetLastError());
+  return MPKX_OK;
+}
+
+int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots, long *row_levels,
+                       long *matrix_bytes, int *values, int *launches) {
+  clear_error();
+  if (!f) return fail(MPKX_EINVAL, "fused32_stats: null");
+  if (max_slots) *max_slots = f->plan.maxslots;
+  if (slots) *slots = f->slots;
+  if (row_levels) *row_levels = f->plan.row_levels;
+  if (matrix_bytes) *matrix_bytes = f->matrix_bytes;
+  if (values) *values = f->values;
+  if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
+  return MPKX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,383 @@
+// fused_impl.h -- the fused "skirt round" matrix powers kernel and its tile
+// plan, shared by the FP64 form (dev_fused.hip) and the FP32/BF16-storage
+// form (dev_fused32.hip).
+//
+// Schedule.  The reference's nphase = 0 schedule (mpk2/driver.c:290-311,
+// mpk2/buffers.c:155-179) gives every partition p its skirt: all rows j
+// within graph distance d = dist(j, p) < k, and p computes (level l, row j)
+// for l <= k - d without any communication.  Here the same construction is
+// applied every s levels ("a round"): for levels L0+1 .. L0+w (w <= s) a
+// workgroup owning tile p
+//   1. gathers level-L0 values of its skirt region (d <= w) from the Krylov
+//      block in HBM into LDS -- this replaces the MPI_Alltoallv halo
+//      exchange between phases;
+//   2. for j = 1..w computes level L0+j of every region row with
+//      d <= w - j from the LDS copy of level L0+j-1 (ping-pong buffers),
+//      one barrier between levels;
+//   3. writes the tile's own rows (d == 0) of every level to HBM.
+// Redundant work on the shrinking skirt is traded for halo traffic; s is the
+// knob (the reference's "levels per phase").
+//
+// Per-tile layout, built once on the host:
+//   slots   region rows sorted by (distance, row): rows active at relative
+//           level j are the prefix cnt[w-j]; the tile's own rows come first.
+//   matrix  compute rows (d <= s-1) as a column-major ELL block of W entries
+//           per row, two 16-bit entries per 32-bit word, [m][word][thread]:
+//           row r = m*T + thread lives in thread r % T's registers for the
+//           whole launch.  An entry is a 14-bit slot plus a 2-bit value code
+//           (dictionary of <= 4 distinct values, passed as kernel arguments),
+//           or the slot alone with a value stream [m][w][thread] (FP64,
+//           FP32 or BF16).  Padding entries are 0xffff and skipped.
+#pragma once
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+
+#include "dev_common.h"
+
+namespace mpkx {
+
+constexpr int FUSED_HDR = 16;       // ints per tile header
+constexpr int FUSED_MAX_S = 10;     // levels per launch (header holds cnt[0..s])
+constexpr uint32_t FUSED_PAD16 = 0xffffu;
+constexpr int FUSED_MAX_SLOTS = 16383;  // 14-bit slot field
+constexpr long FUSED_LDS_BYTES = 160 * 1024;
+
+// value of a stored matrix entry
+__device__ __forceinline__ double mval_of(double v) { return v; }
+__device__ __forceinline__ float mval_of(float v) { return v; }
+__device__ __forceinline__ float mval_of(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+
+// one multiply-accumulate in the accumulation precision.  FP64 rounds the
+// product and the sum separately (the reference's operation sequence);
+// FP32 uses a fused multiply-add.
+__device__ __forceinline__ double mac(double s, double a, double x) { return __dadd_rn(s, __dmul_rn(a, x)); }
+__device__ __forceinline__ float mac(float s, float a, float x) { return fmaf(a, x, s); }
+
+// VT: vector / accumulation type (double or float).  MT: stored matrix
+// value type when DICT is false (double, float or uint16_t = bf16).
+// inv_sigma scales every new power (power of two: exact); FP64 passes 1.
+template <typename VT, typename MT, int T, int MR, int W, bool DICT>
+__global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
+                                             const uint32_t *__restrict__ ent, const MT *__restrict__ vals,
+                                             VT d0, VT d1, VT d2, VT d3, VT inv_sigma, VT *vv, int n,
+                                             int L0, int w, int ntiles, int maxslots) {
+  constexpr int W2 = (W + 1) / 2;
+  constexpr bool SCALE = std::is_same<VT, float>::value;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + maxslots;
+  const int tile = xcd_remap(blockIdx.x, ntiles);
+  const int *h = hdr + (size_t)tile * FUSED_HDR;
+  const int *g = grow + h[0];
+  const size_t eo = (size_t)(unsigned)h[1];
+  const size_t vo = (size_t)(unsigned)h[2];
+  const int *cnt = h + 4;
+  const int tid = threadIdx.x;
+
+  // gather: four slots per thread in flight at a time (row indices, then
+  // values, then LDS stores) so each wave keeps several loads outstanding
+  const int nin = cnt[w];
+  const VT *src = vv + (size_t)L0 * n;
+  for (int t0 = tid; t0 < nin; t0 += 4 * T) {
+    int gi[4];
+    VT gv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] : -1;
+#pragma unroll
+    for (int u = 0; u < 4; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (gi[u] >= 0) buf0[t0 + u * T] = gv[u];
+  }
+
+  const int ncomp = cnt[w - 1], nown = cnt[0];
+  uint32_t e[MR][W2];
+  int own[MR];
+#pragma unroll
+  for (int m = 0; m < MR; m++) {
+    const int r = m * T + tid;
+#pragma unroll
+    for (int k = 0; k < W2; k++) e[m][k] = r < ncomp ? ent[eo + (size_t)(m * W2 + k) * T + tid] : 0xffffffffu;
+    own[m] = r < nown ? g[r] : -1;
+  }
+  __syncthreads();
+
+  VT *in = buf0, *out = buf1;
+  for (int j = 1; j <= w; j++) {
+    const int nact = cnt[w - j];
+    VT *dst = vv + (size_t)(L0 + j) * n;
+    // value stream re-read per level (L2 resident): keep its loads in the loop
+    const MT *vp = vals;
+    asm volatile("" : "+s"(vp));
+#pragma unroll
+    for (int m = 0; m < MR; m++) {
+      const int r = m * T + tid;
+      if (r < nact) {
+        // The packed words are loop invariant; passing them through an empty
+        // asm makes the compiler decode them here, per level, instead of
+        // hoisting MR*W decoded values and addresses into registers.
+        uint32_t ew[W2];
+#pragma unroll
+        for (int k = 0; k < W2; k++) {
+          ew[k] = e[m][k];
+          asm volatile("" : "+v"(ew[k]));
+        }
+        VT s = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+          const uint32_t x = (k & 1) ? (ew[k >> 1] >> 16) : (ew[k >> 1] & 0xffffu);
+          if (x != FUSED_PAD16) {
+            VT a;
+            if (DICT) {
+              const uint32_t c = x >> 14;
+              a = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
+            } else {
+              a = (VT)mval_of(vp[vo + (size_t)(m * W + k) * T + tid]);
+            }
+            s = mac(s, a, in[x & 0x3fffu]);
+          }
+        }
+        if (SCALE) s = s * inv_sigma;
+        out[r] = s;
+        if (own[m] >= 0) dst[own[m]] = s;
+      }
+      // keep the next row's LDS loads below this point: one row's operands
+      // live at a time instead of MR*W of them
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    VT *t = in;
+    in = out;
+    out = t;
+  }
+}
+
+template <typename VT, typename MT>
+using FusedKernel = void (*)(const int *, const int *, const uint32_t *, const MT *, VT, VT, VT, VT, VT,
+                             VT *, int, int, int, int, int);
+
+template <typename VT, typename MT, int T, int MR>
+FusedKernel<VT, MT> pick_fused_w(int W, bool dict) {
+#define MPKX_FUSED_CASE(WW) \
+  case WW:                  \
+    return dict ? k_fused<VT, MT, T, MR, WW, true> : k_fused<VT, MT, T, MR, WW, false>;
+  switch (W) {
+    MPKX_FUSED_CASE(5)
+    MPKX_FUSED_CASE(7)
+    MPKX_FUSED_CASE(9)
+    MPKX_FUSED_CASE(16)
+  }
+#undef MPKX_FUSED_CASE
+  return nullptr;
+}
+
+// Host-side tile plan.
+struct TilePlan {
+  int n = 0, nlevel = 0, s = 0, ntiles = 0, W = 0, T = 0, MR = 0, maxslots = 0;
+  bool dict = false;
+  std::vector<double> dictv;  // up to 4 distinct values
+  std::vector<int> hdr, grow;
+  std::vector<uint32_t> ent;
+  std::vector<double> vals;   // value stream (explicit codec), [m][k][T] per tile
+  long row_levels = 0;
+};
+
+// Builds the plan.  slot_bytes: LDS bytes per region row (2 buffers).
+// mr_choices: allowed (T, MR) pairs, smallest first.
+inline int build_tile_plan(int n, const int *ptr, const int *col, const double *val, const int *part,
+                           int npart, int nlevel, int s, bool allow_dict, int slot_bytes,
+                           const std::vector<std::pair<int, int>> &mr_choices, TilePlan &P) {
+  if (n <= 0 || !ptr || !col || !val || !part || npart <= 0 || nlevel <= 0)
+    return fail(MPKX_EINVAL, "fused: bad arguments");
+  if (s < 1 || s > FUSED_MAX_S) return fail(MPKX_EINVAL, "fused: s=%d outside 1..%d", s, FUSED_MAX_S);
+  s = std::min(s, nlevel);
+  int maxdeg = 0;
+  for (int i = 0; i < n; i++) maxdeg = std::max(maxdeg, ptr[i + 1] - ptr[i]);
+  int W = maxdeg <= 5 ? 5 : maxdeg <= 7 ? 7 : maxdeg <= 9 ? 9 : maxdeg <= 16 ? 16 : 0;
+  if (!W) return fail(MPKX_EUNSUPPORTED, "fused: rows up to %d entries, at most 16 supported", maxdeg);
+  for (long j = 0; j < ptr[n]; j++)
+    if (col[j] < 0 || col[j] >= n) return fail(MPKX_EINVAL, "fused: column out of range at %ld", j);
+
+  std::vector<double> dict;
+  bool use_dict = allow_dict;
+  for (long j = 0; j < ptr[n] && use_dict; j++) {
+    bool found = false;
+    for (double d : dict)
+      if (memcmp(&d, &val[j], sizeof(double)) == 0) found = true;
+    if (!found) {
+      if (dict.size() == 4)
+        use_dict = false;
+      else
+        dict.push_back(val[j]);
+    }
+  }
+  if (!use_dict) dict.clear();
+
+  std::vector<std::vector<int>> members(npart);
+  for (int i = 0; i < n; i++) {
+    if (part[i] < 0 || part[i] >= npart) return fail(MPKX_EINVAL, "fused: label %d out of range", part[i]);
+    members[part[i]].push_back(i);
+  }
+  for (int p = 0; p < npart; p++)
+    if (members[p].empty()) return fail(MPKX_EINVAL, "fused: partition %d is empty", p);
+
+  struct Tile {
+    std::vector<int> slots, cnt, rows;
+    std::vector<double> vals;
+  };
+  std::vector<Tile> tiles(npart);
+  std::atomic<long> next{0};
+  int nthreads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  auto worker = [&]() {
+    std::vector<int> dist(n, -1), slot(n, -1);
+    for (;;) {
+      long p = next.fetch_add(1);
+      if (p >= npart) break;
+      Tile &t = tiles[p];
+      std::vector<int> q(members[p]);
+      for (int i : q) dist[i] = 0;
+      for (size_t hh = 0; hh < q.size(); hh++) {
+        int u = q[hh];
+        if (dist[u] >= s) continue;
+        for (int j = ptr[u]; j < ptr[u + 1]; j++)
+          if (dist[col[j]] < 0) {
+            dist[col[j]] = dist[u] + 1;
+            q.push_back(col[j]);
+          }
+      }
+      std::sort(q.begin(), q.end(), [&](int a, int b) { return dist[a] != dist[b] ? dist[a] < dist[b] : a < b; });
+      t.slots = q;
+      t.cnt.assign(s + 1, 0);
+      for (int u : q) t.cnt[dist[u]]++;
+      for (int d = 1; d <= s; d++) t.cnt[d] += t.cnt[d - 1];
+      for (size_t k = 0; k < q.size(); k++) slot[q[k]] = (int)k;
+      int ncomp = t.cnt[s - 1];
+      t.rows.assign((size_t)ncomp * W, -1);
+      t.vals.assign((size_t)ncomp * W, 0.0);
+      for (int r = 0; r < ncomp; r++) {
+        int u = q[r];
+        for (int j = ptr[u]; j < ptr[u + 1]; j++) {
+          t.rows[(size_t)r * W + (j - ptr[u])] = slot[col[j]];
+          t.vals[(size_t)r * W + (j - ptr[u])] = val[j];
+        }
+      }
+      for (int u : q) dist[u] = -1, slot[u] = -1;
+    }
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int i = 0; i < nthreads; i++) pool.emplace_back(worker);
+    for (auto &th : pool) th.join();
+  }
+  int maxslots = 0, maxcomp = 0;
+  for (auto &t : tiles) {
+    maxslots = std::max(maxslots, (int)t.slots.size());
+    maxcomp = std::max(maxcomp, t.cnt[s - 1]);
+  }
+  if (maxslots > FUSED_MAX_SLOTS)
+    return fail(MPKX_EUNSUPPORTED, "fused: %d region rows exceed %d slots", maxslots, FUSED_MAX_SLOTS);
+  if ((long)maxslots * slot_bytes > FUSED_LDS_BYTES)
+    return fail(MPKX_EUNSUPPORTED,
+                "fused: tile region of %d rows needs %ld B LDS (> %ld); use more partitions or a smaller s",
+                maxslots, (long)maxslots * slot_bytes, FUSED_LDS_BYTES);
+  int T = 0, MR = 0;
+  for (auto &c : mr_choices)  // register budget: at most 64 packed words per thread
+    if (maxcomp <= c.first * c.second && c.second * ((W + 1) / 2) <= 64) {
+      T = c.first;
+      MR = c.second;
+      break;
+    }
+  if (!T) return fail(MPKX_EUNSUPPORTED, "fused: %d compute rows per tile exceed the register blocking", maxcomp);
+
+  P.n = n;
+  P.nlevel = nlevel;
+  P.s = s;
+  P.ntiles = npart;
+  P.W = W;
+  P.T = T;
+  P.MR = MR;
+  P.maxslots = maxslots;
+  P.dict = use_dict;
+  P.dictv = dict;
+  P.hdr.assign((size_t)npart * FUSED_HDR, 0);
+  P.grow.clear();
+  P.ent.clear();
+  P.vals.clear();
+  const int W2 = (W + 1) / 2;
+  long slot_off = 0, ent_off = 0, val_off = 0;
+  for (int p = 0; p < npart; p++) {
+    const Tile &t = tiles[p];
+    int ncomp = t.cnt[s - 1];
+    int mrt = (ncomp + T - 1) / T;
+    int *h = &P.hdr[(size_t)p * FUSED_HDR];
+    h[0] = (int)slot_off;
+    h[1] = (int)ent_off;
+    h[2] = (int)val_off;
+    h[3] = (int)t.slots.size();
+    for (int d = 0; d <= s; d++) h[4 + d] = t.cnt[d];
+    P.grow.insert(P.grow.end(), t.slots.begin(), t.slots.end());
+    size_t eseg = (size_t)mrt * W2 * T, vseg = use_dict ? 0 : (size_t)mrt * W * T;
+    P.ent.resize(ent_off + eseg, 0xffffffffu);
+    P.vals.resize(val_off + vseg, 0.0);
+    for (int r = 0; r < ncomp; r++) {
+      int m = r / T, th = r % T;
+      for (int k = 0; k < W; k++) {
+        int sl = t.rows[(size_t)r * W + k];
+        if (sl < 0) continue;
+        uint32_t code = 0;
+        double v = t.vals[(size_t)r * W + k];
+        if (use_dict) {
+          for (size_t c = 0; c < dict.size(); c++)
+            if (memcmp(&dict[c], &v, sizeof(double)) == 0) code = (uint32_t)c;
+        } else {
+          P.vals[val_off + (size_t)(m * W + k) * T + th] = v;
+        }
+        uint32_t e16 = (uint32_t)sl | (code << 14);
+        uint32_t &word = P.ent[ent_off + (size_t)(m * W2 + k / 2) * T + th];
+        word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
+      }
+    }
+    slot_off += (long)t.slots.size();
+    ent_off += (long)eseg;
+    val_off += (long)vseg;
+    if (ent_off > INT32_MAX || slot_off > INT32_MAX || val_off > INT32_MAX)
+      return fail(MPKX_EUNSUPPORTED, "fused: plan exceeds 32-bit offsets");
+  }
+  long rl = 0;
+  for (int L0 = 0; L0 < nlevel; L0 += s) {
+    int w = std::min(s, nlevel - L0);
+    for (auto &t : tiles)
+      for (int j = 1; j <= w; j++) rl += t.cnt[w - j];
+  }
+  P.row_levels = rl;
+  return MPKX_OK;
+}
+
+// Records `launch(stream)` into a hipGraph keyed by (a, b, stream) and replays it.
+template <class F>
+int graph_run(GraphCache &gc, const void *a, const void *b, hipStream_t st, F launch) {
+  if (!gc.matches(a, b, st, 0)) {
+    gc.reset();
+    hipStream_t cap = st;
+    bool own = false;
+    if (cap == nullptr) {  // the legacy default stream cannot be captured
+      MPKX_HIP(hipStreamCreate(&cap));
+      own = true;
+    }
+    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    launch(cap);
+    MPKX_HIP(hipStreamEndCapture(cap, &gc.graph));
+    MPKX_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+    if (own) MPKX_HIP(hipStreamDestroy(cap));
+    gc.k0 = a;
+    gc.k1 = b;
+    gc.stream = st;
+    gc.k = 0;
+  }
+  MPKX_HIP(hipGraphLaunch(gc.exec, st));
+  return MPKX_OK;
+}
+
+}  // namespace mpkx

@@ -1,0 +1,75 @@
+"""Exhaustive autotuner for the fused matrix powers kernel.
+
+For a matrix and k it times every candidate (partition, levels per launch s,
+precision) on the GPU and returns the measurements sorted by time.  A
+candidate whose tile region does not fit the LDS or register budget is
+reported as unsupported.  This is the judge that any learned cost model is
+checked against.
+
+Candidates for grid matrices are Cartesian block partitions (tile shapes);
+for general matrices the graph partitioner with a range of part counts.
+"""
+import itertools
+import time
+
+import numpy as np
+
+import mpk_amd as M
+
+
+def grid_candidates(dims, tile_sides=(8, 12, 16, 24, 32), s_values=(1, 2, 3, 4, 5, 6)):
+    """(blocks per dim, s) pairs for tiles with sides from tile_sides."""
+    out = []
+    for sides in itertools.product(tile_sides, repeat=len(dims)):
+        if any(d % t for d, t in zip(dims, sides)):
+            continue
+        if list(sides) != sorted(sides, reverse=True):  # one orientation per shape
+            continue
+        blocks = [d // t for d, t in zip(dims, sides)]
+        for s in s_values:
+            out.append((tuple(blocks), s))
+    return out
+
+
+def time_fused(F, xd, vv, reps, warmup=2):
+    import torch
+    for _ in range(warmup):
+        F.exec(xd, vv)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        F.exec(xd, vv)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, reps=10, x=None, log=None):
+    """Times every (blocks, s) candidate; returns a list of dicts sorted by ms."""
+    import torch
+    n = len(ptr) - 1
+    if x is None:
+        x = M.uniform_pm1(0x5EED, n)
+    xd = torch.from_numpy(x).to("cuda:0")
+    dt = torch.float64 if precision == "f64" else torch.float32
+    vv = torch.zeros(n * (nlevel + 1), dtype=dt, device="cuda:0")
+    res = []
+    for blocks, s in (candidates or grid_candidates(dims)):
+        rec = {"blocks": list(blocks), "s": s, "precision": precision}
+        t0 = time.perf_counter()
+        try:
+            part = M.partition_grid(dims, blocks)
+            F = (M.FusedMPK(ptr, col, val, part, nlevel, s) if precision == "f64"
+                 else M.FusedMPK32(ptr, col, val, part, nlevel, s))
+            rec["ms"] = time_fused(F, xd, vv, reps)
+            rec.update({k: v for k, v in F.stats().items() if k in ("max_slots", "row_levels", "matrix_bytes")})
+            del F
+        except M.MpkxError as e:
+            rec["unsupported"] = str(e)
+        rec["build_s"] = round(time.perf_counter() - t0, 3)
+        res.append(rec)
+        if log:
+            log(rec)
+    ok = sorted([r for r in res if "ms" in r], key=lambda r: r["ms"])
+    return ok + [r for r in res if "ms" not in r]

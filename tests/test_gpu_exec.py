@@ -151,3 +151,68 @@ def test_exact_plan_cube64_against_oracle():
         vv = zeros_dev(n * (k + 1))
         plan.exec(to_dev(x), vv)
         assert np.array_equal(to_host(vv), gold)
+
+
+def _rel_errs_scaled(out32, gold, n, k, sigma):
+    errs = []
+    for l in range(k + 1):
+        a = out32[l * n:(l + 1) * n].astype(np.float64) * sigma ** l
+        b = gold[l * n:(l + 1) * n]
+        errs.append(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("values,tol", [("auto", 1e-5), ("fp32", 1e-5)])
+def test_fused32_cube_per_power_error(values, tol):
+    """FP32 form on the 7-pt Laplacian (values exact in FP32): error vs the FP64
+    oracle stays at FP32 rounding level for every power."""
+    import torch
+    say("fused32 cube", values)
+    N, k = 64, 8
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    n = len(rp) - 1
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    part = M.partition_grid([N] * 3, [4, 4, 4])
+    F = M.FusedMPK32(rp, ci, va, part, k, 4, {"auto": 0, "fp32": 1}[values])
+    assert F.sigma == 16.0
+    vv = torch.zeros(n * (k + 1), dtype=torch.float32, device="cuda")
+    F.exec(to_dev(x), vv)
+    out = to_host(vv)
+    errs = _rel_errs_scaled(out, gold, n, k, F.sigma)
+    say("per-power error", errs)
+    assert errs.max() <= tol
+    # FP64 norms on the GPU equal the host norms of the FP32 block
+    nr = M.norms32(vv, n, k)
+    host = [np.linalg.norm(out[l * n:(l + 1) * n].astype(np.float64)) for l in range(k + 1)]
+    np.testing.assert_allclose(nr, host, rtol=1e-12)
+    A = M.CsrMatrix(rp, ci, va)
+    res = M.residual32(A, vv, k, F.sigma)
+    assert res < 1e-6
+
+
+def test_fused32_bf16_random_values_stated_error():
+    """BF16 matrix storage on a 9-pt stencil with random off-diagonals: the
+    per-power error is dominated by the BF16 rounding of A (about 2^-9)."""
+    import torch
+    say("fused32 bf16")
+    N, k = 256, 16
+    rp, ci, va = M.gen2("m9p", N, -1.0)
+    n = len(rp) - 1
+    rng = np.random.default_rng(5)
+    off = ci != np.repeat(np.arange(n), np.diff(rp))
+    va = va.copy()
+    va[off] = -1.0 * (1 + 0.1 * rng.standard_normal(off.sum()))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    absum = np.bincount(rows, weights=np.abs(va) * off, minlength=n)
+    va[~off] = absum + 0.5
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    part = M.partition_grid([N, N], [8, 8])
+    for values, tol in ((1, 1e-5), (2, 3e-2)):
+        F = M.FusedMPK32(rp, ci, va, part, k, 4, values)
+        vv = torch.zeros(n * (k + 1), dtype=torch.float32, device="cuda")
+        F.exec(to_dev(x), vv)
+        errs = _rel_errs_scaled(to_host(vv), gold, n, k, F.sigma)
+        say("values", values, "sigma", F.sigma, "per-power error", errs)
+        assert errs.max() <= tol
