@@ -109,6 +109,22 @@ def cpu_baseline_reference(M, rp, ci, va, x, N, k, gold_check):
             "results_bitwise_equal_gpu": ok}
 
 
+def measured_traffic(N, k, s, blocks):
+    """HBM bytes per launch of the dominant fused round, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this configuration (None if
+    this configuration was not profiled)."""
+    path = os.path.join(REPO, "profiles", "k_fused_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    cfg = rec.get("config", {})
+    if cfg.get("N") == N and cfg.get("k") == k and cfg.get("s") == s and cfg.get("blocks") == blocks:
+        return rec.get("hbm_bytes_per_launch")
+    return None
+
+
 def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
     """k back-to-back SpMVs through torch.sparse CSR (hipSPARSE/rocSPARSE on
     ROCm), as an outside reference point for the plain kernel."""
@@ -139,10 +155,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--k", type=int, default=8)
-    ap.add_argument("--s", type=int, default=3, help="levels per fused launch")
-    ap.add_argument("--blocks", type=int, default=8, help="tiles per dimension")
-    ap.add_argument("--s32", type=int, default=4, help="levels per launch, FP32 form")
-    ap.add_argument("--blocks32", type=int, default=8, help="tiles per dimension, FP32 form")
+    ap.add_argument("--s", type=int, default=2, help="levels per fused launch")
+    ap.add_argument("--blocks", default="8,8,16",
+                    help="tiles per dimension (slowest first); default from the autotuner sweep")
+    ap.add_argument("--s32", type=int, default=2, help="levels per launch, FP32 form")
+    ap.add_argument("--blocks32", default="8,16,16", help="tiles per dimension, FP32 form")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run one 512 MiB k_dcopy (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cold-reps", type=int, default=10, help="cold-MALL repetitions (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -159,7 +178,9 @@ def main():
     rp, ci, va = M.gen2("cube", N, -1.0)
     n, nnz = len(rp) - 1, len(ci)
     x = M.uniform_pm1(0x5EED, n)
-    part = M.partition_grid([N, N, N], [args.blocks] * 3)
+    blocks = [int(b) for b in args.blocks.split(",")]
+    blocks32 = [int(b) for b in args.blocks32.split(",")]
+    part = M.partition_grid([N, N, N], blocks)
     F = M.FusedMPK(rp, ci, va, part, k, s)
     A = M.CsrMatrix(rp, ci, va)
     dev = torch.device("cuda:0")
@@ -234,7 +255,7 @@ def main():
     # (dictionary), per-power scaling by 1/sigma
     fp32 = None
     try:
-        part32 = M.partition_grid([N, N, N], [args.blocks32] * 3)
+        part32 = M.partition_grid([N, N, N], blocks32)
         F32 = M.FusedMPK32(rp, ci, va, part32, k, args.s32)
         v32 = torch.zeros(n * (k + 1), dtype=torch.float32, device=dev)
         for _ in range(args.warmup):
@@ -247,7 +268,7 @@ def main():
         fp32 = {"ms_per_step": round(ms32, 5), "value": round(k * algorithmic_bytes(n, nnz) / (ms32 * 1e-3) / 1e9, 2),
                 "unit": "GB/s (FP64-equivalent bytes)", "speedup_vs_kxspmv_fp64": round(ms_plain / ms32, 3),
                 "per_power_rel_err_vs_fp64": [float("%.3e" % e) for e in errs],
-                "levels_per_launch": args.s32, "tiles": args.blocks32 ** 3, "plan": F32.stats()}
+                "levels_per_launch": args.s32, "tile_blocks": blocks32, "plan": F32.stats()}
         if cold_ms is not None:
             cold_ms["fused_fp32"] = cold(lambda: F32.exec(xd, v32), args.cold_reps)
         del v32, F32
@@ -260,6 +281,13 @@ def main():
         ms_rs = None
         print(f"rocSPARSE reference point unavailable: {e}", file=sys.stderr)
 
+    if args.calib:  # known-size streaming copy for the counter calibration
+        big = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        big2 = torch.empty_like(big)
+        M.dcopy(big2, big, big.numel())
+        torch.cuda.synchronize()
+        del big, big2
+
     B = algorithmic_bytes(n, nnz)
     gbs_fused = k * B / (ms_fused * 1e-3) / 1e9
     gbs_plain = k * B / (ms_plain * 1e-3) / 1e9
@@ -267,6 +295,7 @@ def main():
     r_dom = int(np.argmax(round_ms))
     w_dom = min(s, k - r_dom * s)
     achieved = w_dom * B / (round_ms[r_dom] * 1e-3) / 1e9
+    traffic = measured_traffic(N, k, s, blocks)
 
     out = {
         "metric": "MPK effective GB/s & % HBM roofline, fused vs kxSpMV, 3D 7-pt k=8",
@@ -282,9 +311,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (generated 7-pt Laplacian, seeded x)",
         "config": {"workload": f"3D 7-pt Laplacian {N}^3 (n={n}, nnz={nnz}), k={k}, FP64, "
-                               f"fused skirt rounds s={s}, {args.blocks ** 3} tiles",
+                               f"fused skirt rounds s={s}, {int(np.prod(blocks))} tiles",
                    "n": n, "nnz": nnz, "k": k, "levels_per_launch": s,
-                   "tiles": args.blocks ** 3, "parallelism": "single GPU"},
+                   "tile_blocks": blocks, "parallelism": "single GPU"},
         "pct_hbm_roofline": round(100 * gbs_fused / HBM_PEAK_GBS, 2),
         "baseline_kxspmv": {"value": round(gbs_plain, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 5),
                             "pct_hbm_roofline": round(100 * gbs_plain / HBM_PEAK_GBS, 2)},
@@ -297,7 +326,7 @@ def main():
         "parity_bitwise_fused_vs_kxspmv": parity,
         "fused_plan": st,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_fused (one round of s levels)",
                      "round_ms": [round(t, 5) for t in round_ms]},
     }

@@ -204,10 +204,30 @@ int mpkx_exec_bufs(mpkx_plan *plan, mpkx_buffers *const *bufs);
  * loading rbuf[0] = x[idx_buf] and collecting like collect_results
  * (mpk2/mpi2_mpkexec.c:122-146). */
 int mpkx_exec(mpkx_plan *plan, const double *x, double *vv, void *stream);
+/* Device-to-device copy of n doubles by a plain streaming kernel
+ * (k_dcopy: one load and one store per element).  Used to calibrate HBM
+ * traffic counters against a kernel of known size. */
+int mpkx_dcopy(double *dst, const double *src, long n, void *stream);
 /* Statistics: rows computed, matrix entries used, ghost values moved,
  * kernel launches per exec. */
 int mpkx_plan_stats(const mpkx_plan *plan, long *rows, long *nnz,
                     long *ghosts, int *launches);
+
+/* ------------------------------------------------------------------ */
+/* Shared-memory exec of prep_mpk task lists (mpk2/mpkexec.c:10-130,   */
+/* valued form mpk2/mpkvexec.c:10-55): one shared Krylov block; phases */
+/* in order, the (phase, partition) lists level-major.  val == NULL    */
+/* gives exec_mpk's 1/deg(i) weights (mpk2/mpkexec.c:28).  tcount and  */
+/* tidx are mpkx_prep_mpk's output ((nphase+1)*npart lists, back to    */
+/* back).  FP64 results are bitwise equal to vexec_mpk / k x SpMV.     */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_tasks mpkx_tasks;
+int mpkx_tasks_create(mpkx_tasks **t, int n, const int *ptr, const int *col,
+                      const double *val, int nlevel, int nphase, int npart,
+                      const int *tcount, const long *tidx);
+void mpkx_tasks_destroy(mpkx_tasks *t);
+/* x, vv device; vv[0..n) receives x, then every task l*n+i is computed. */
+int mpkx_tasks_exec(mpkx_tasks *t, const double *x, double *vv, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Plain k x SpMV baseline (mpk2/verify_val.c:44-75): vv[(l+1)n ..] =   */

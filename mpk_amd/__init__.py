@@ -95,6 +95,10 @@ def lib():
         "mpkx_exec_bufs": ([P, P], C.c_int),
         "mpkx_exec": ([P, P, P, P], C.c_int),
         "mpkx_plan_stats": ([P, P, P, P, P], C.c_int),
+        "mpkx_dcopy": ([P, P, C.c_long, P], C.c_int),
+        "mpkx_tasks_create": ([P, C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, P, P], C.c_int),
+        "mpkx_tasks_destroy": ([P], None),
+        "mpkx_tasks_exec": ([P, P, P, P], C.c_int),
         "mpkx_csr_create": ([P, C.c_int, P, P, P], C.c_int),
         "mpkx_csr_destroy": ([P], None),
         "mpkx_spmv_k": ([P, P, C.c_int, P, C.c_int], C.c_int),
@@ -450,6 +454,26 @@ class Plan:
             self._h = None
 
 
+class TaskPlan:
+    """GPU exec of prep_mpk task lists over one shared Krylov block
+    (mpk2/mpkexec.c, mpk2/mpkvexec.c); val=None gives the 1/deg weights."""
+
+    def __init__(self, ptr, col, val, nlevel, nphase, npart, tcount, tidx):
+        self._h = C.c_void_p()
+        va = None if val is None else _p(np.ascontiguousarray(val, np.float64))
+        _check(lib().mpkx_tasks_create(C.byref(self._h), len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)), va,
+                                       nlevel, nphase, npart, _p(_i32(tcount)),
+                                       _p(np.ascontiguousarray(tidx, np.int64))))
+
+    def exec(self, x, vv, stream=None):
+        _check(lib().mpkx_tasks_exec(self._h, _dev(x), _dev(vv), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_tasks_destroy(self._h)
+            self._h = None
+
+
 class CsrMatrix:
     """Device CSR matrix for the plain k x SpMV baseline."""
 
@@ -561,6 +585,11 @@ class FusedMPK32:
         if getattr(self, "_h", None):
             lib().mpkx_fused32_destroy(self._h)
             self._h = None
+
+
+def dcopy(dst, src, n, stream=None):
+    """Plain streaming device copy of n doubles (HBM counter calibration)."""
+    _check(lib().mpkx_dcopy(_dev(dst), _dev(src), n, _stream(stream)))
 
 
 def norms32(vv, n, nlevel, stream=None):

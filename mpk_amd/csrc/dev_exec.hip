@@ -65,6 +65,11 @@ __global__ void k_copy_pairs(double *flat, const int *__restrict__ dst, const in
   if (t < count) flat[dst[t]] = flat[src[t]];
 }
 
+__global__ void k_dcopy(double *__restrict__ dst, const double *__restrict__ src, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
 __global__ void k_load_pairs(double *__restrict__ out, const double *__restrict__ in,
                              const int *__restrict__ dst, const int *__restrict__ src, long count) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,6 +283,15 @@ int mpkx_plan_create(mpkx_plan **out, mpkx_buffers *const *bufs, int npart) {
 
 void mpkx_plan_destroy(mpkx_plan *plan) { delete plan; }
 
+int mpkx_dcopy(double *dst, const double *src, long n, void *stream) {
+  clear_error();
+  if (!dst || !src || n < 0) return fail(MPKX_EINVAL, "dcopy: bad arguments");
+  if (n == 0) return MPKX_OK;
+  hipLaunchKernelGGL(k_dcopy, dim3(grid1(n, 256)), dim3(256), 0, (hipStream_t)stream, dst, src, n);
+  MPKX_HIP(hipGetLastError());
+  return MPKX_OK;
+}
+
 int mpkx_plan_stats(const mpkx_plan *pl, long *rows, long *nnz, long *ghosts, int *launches) {
   clear_error();
   if (!pl) return fail(MPKX_EINVAL, "plan_stats: null plan");
@@ -436,6 +450,105 @@ int mpkx_spmv_k(mpkx_csr *m, double *vv, int k, void *stream, int use_graph) {
     m->graph.k = k;
   }
   MPKX_HIP(hipGraphLaunch(m->graph.exec, st));
+  return MPKX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Shared-memory schedule (mpk2/mpkprep.c + mpk2/mpkexec.c / mpkvexec.c):
+// task lists of l*n+i per (phase, partition) over one shared vv.  Phases run
+// in order; inside a phase every partition's tasks are level-major, so one
+// launch per (phase, level) over all partitions honours every dependency.
+// ---------------------------------------------------------------------------
+struct mpkx_tasks {
+  int n = 0, nlevel = 0, nphase = 0, npart = 0;
+  struct Group {
+    int blk0, nblk;
+  };
+  std::vector<Group> groups;
+  DevArray<long> rp;
+  DevArray<int> ci, out_row, blk;
+  DevArray<double> va;
+  long ntask = 0;
+};
+
+extern "C" {
+
+int mpkx_tasks_create(mpkx_tasks **out, int n, const int *ptr, const int *col, const double *val,
+                      int nlevel, int nphase, int npart, const int *tcount, const long *tidx) {
+  clear_error();
+  if (!out || n <= 0 || !ptr || !col || nlevel <= 0 || nphase < 0 || npart <= 0 || !tcount || !tidx)
+    return fail(MPKX_EINVAL, "tasks_create: bad arguments");
+  if ((long)n * (nlevel + 1) > INT32_MAX) return fail(MPKX_EUNSUPPORTED, "tasks_create: n*(nlevel+1) exceeds int32");
+  auto *t = new (std::nothrow) mpkx_tasks;
+  if (!t) return fail(MPKX_ENOMEM, "tasks_create");
+  std::unique_ptr<mpkx_tasks> guard(t);
+  t->n = n;
+  t->nlevel = nlevel;
+  t->nphase = nphase;
+  t->npart = npart;
+  // list start offsets (lists are stored back to back: phase-major, partition-minor)
+  std::vector<long> start((size_t)(nphase + 1) * npart + 1, 0);
+  for (size_t q = 0; q + 1 < start.size(); q++) start[q + 1] = start[q] + tcount[q];
+  t->ntask = start.back();
+  std::vector<long> rp(1, 0);
+  std::vector<int> ci, outr, blk;
+  std::vector<double> va;
+  try {
+    for (int ph = 0; ph <= nphase; ph++) {
+      std::vector<long> cur(npart);
+      for (int p = 0; p < npart; p++) cur[p] = start[(size_t)ph * npart + p];
+      for (int l = 1; l <= nlevel; l++) {
+        int r0 = (int)rp.size() - 1;
+        for (int p = 0; p < npart; p++) {
+          long end = start[(size_t)ph * npart + p + 1];
+          long &c = cur[p];
+          for (; c < end && tidx[c] / n == l; c++) {
+            long i = tidx[c] % n;
+            double inv = 1.0 / (ptr[i + 1] - ptr[i]);  // exec_mpk's 1/deg (mpk2/mpkexec.c:28)
+            for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+              ci.push_back((int)((long)(l - 1) * n + col[j]));
+              va.push_back(val ? val[j] : inv);
+            }
+            rp.push_back((long)ci.size());
+            outr.push_back((int)tidx[c]);
+          }
+        }
+        int r1 = (int)rp.size() - 1;
+        if (r1 > r0) {
+          int b0 = (int)blk.size();
+          make_blocks(rp, r0, r1, blk);
+          t->groups.push_back({b0, (int)blk.size() - b0});
+        }
+      }
+      for (int p = 0; p < npart; p++)
+        if (cur[p] != start[(size_t)ph * npart + p + 1])
+          return fail(MPKX_EINVAL, "tasks_create: phase %d list %d is not level-major", ph, p);
+    }
+    blk.push_back((int)rp.size() - 1);
+  } catch (const std::bad_alloc &) {
+    return fail(MPKX_ENOMEM, "tasks_create: host allocation");
+  }
+  int rc;
+  if ((rc = t->rp.upload(rp)) || (rc = t->ci.upload(ci)) || (rc = t->va.upload(va)) ||
+      (rc = t->out_row.upload(outr)) || (rc = t->blk.upload(blk)))
+    return rc;
+  *out = guard.release();
+  return MPKX_OK;
+}
+
+void mpkx_tasks_destroy(mpkx_tasks *t) { delete t; }
+
+int mpkx_tasks_exec(mpkx_tasks *t, const double *x, double *vv, void *stream) {
+  clear_error();
+  if (!t || !x || !vv) return fail(MPKX_EINVAL, "tasks_exec: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  MPKX_HIP(hipMemcpyAsync(vv, x, sizeof(double) * t->n, hipMemcpyDeviceToDevice, st));
+  for (const auto &g : t->groups)
+    hipLaunchKernelGGL((k_csr_stream<long, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, t->rp.p, t->ci.p,
+                       t->va.p, vv, vv, t->out_row.p, t->blk.p, g.blk0);
+  MPKX_HIP(hipGetLastError());
   return MPKX_OK;
 }
 

@@ -21,12 +21,12 @@
 // Per-tile layout, built once on the host:
 //   slots   region rows sorted by (distance, row): rows active at relative
 //           level j are the prefix cnt[w-j]; the tile's own rows come first.
-//   matrix  compute rows (d <= s-1) as a column-major ELL block of W entries
-//           per row, two 16-bit entries per 32-bit word, [m][word][thread]:
-//           row r = m*T + thread lives in thread r % T's registers for the
-//           whole launch.  An entry is a 14-bit slot plus a 2-bit value code
+//   matrix  compute rows (d <= s-1) as an ELL block of W entries per row,
+//           two 16-bit entries per 32-bit word, the W2 words of a row
+//           consecutive [row][word] (one wide load per row); row r = m*T +
+//           thread lives in thread r % T's registers for the whole launch.  An entry is a 14-bit slot plus a 2-bit value code
 //           (dictionary of <= 4 distinct values, passed as kernel arguments),
-//           or the slot alone with a value stream [m][w][thread] (FP64,
+//           or the slot alone with a value stream [row][w] (FP64,
 //           FP32 or BF16).  Padding entries are 0xffff and skipped.
 #pragma once
 
@@ -76,30 +76,50 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   const int *cnt = h + 4;
   const int tid = threadIdx.x;
 
-  // gather: four slots per thread in flight at a time (row indices, then
+  // gather: GB slots per thread in flight at a time (row indices, then
   // values, then LDS stores) so each wave keeps several loads outstanding
+  constexpr int GB = 8;
   const int nin = cnt[w];
   const VT *src = vv + (size_t)L0 * n;
-  for (int t0 = tid; t0 < nin; t0 += 4 * T) {
-    int gi[4];
-    VT gv[4];
+  for (int t0 = tid; t0 < nin; t0 += GB * T) {
+    int gi[GB];
+    VT gv[GB];
 #pragma unroll
-    for (int u = 0; u < 4; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] : -1;
+    for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] : -1;
 #pragma unroll
-    for (int u = 0; u < 4; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+    for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < GB; u++)
       if (gi[u] >= 0) buf0[t0 + u * T] = gv[u];
   }
 
+  // matrix rows of this thread into registers (W2 consecutive words per
+  // row: one wide load), own-row targets
   const int ncomp = cnt[w - 1], nown = cnt[0];
   uint32_t e[MR][W2];
   int own[MR];
 #pragma unroll
   for (int m = 0; m < MR; m++) {
     const int r = m * T + tid;
+    const uint32_t *ep = ent + eo + (size_t)r * W2;
+    if (r < ncomp) {
+      if constexpr (W2 % 4 == 0) {
 #pragma unroll
-    for (int k = 0; k < W2; k++) e[m][k] = r < ncomp ? ent[eo + (size_t)(m * W2 + k) * T + tid] : 0xffffffffu;
+        for (int q = 0; q < W2 / 4; q++) {
+          const uint4 v4 = reinterpret_cast<const uint4 *>(ep)[q];
+          e[m][4 * q] = v4.x;
+          e[m][4 * q + 1] = v4.y;
+          e[m][4 * q + 2] = v4.z;
+          e[m][4 * q + 3] = v4.w;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < W2; k++) e[m][k] = ep[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < W2; k++) e[m][k] = 0xffffffffu;
+    }
     own[m] = r < nown ? g[r] : -1;
   }
   __syncthreads();
@@ -134,7 +154,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
               const uint32_t c = x >> 14;
               a = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
             } else {
-              a = (VT)mval_of(vp[vo + (size_t)(m * W + k) * T + tid]);
+              a = (VT)mval_of(vp[vo + (size_t)r * W + k]);
             }
             s = mac(s, a, in[x & 0x3fffu]);
           }
@@ -322,7 +342,6 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
     P.ent.resize(ent_off + eseg, 0xffffffffu);
     P.vals.resize(val_off + vseg, 0.0);
     for (int r = 0; r < ncomp; r++) {
-      int m = r / T, th = r % T;
       for (int k = 0; k < W; k++) {
         int sl = t.rows[(size_t)r * W + k];
         if (sl < 0) continue;
@@ -332,10 +351,10 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
           for (size_t c = 0; c < dict.size(); c++)
             if (memcmp(&dict[c], &v, sizeof(double)) == 0) code = (uint32_t)c;
         } else {
-          P.vals[val_off + (size_t)(m * W + k) * T + th] = v;
+          P.vals[val_off + (size_t)r * W + k] = v;
         }
         uint32_t e16 = (uint32_t)sl | (code << 14);
-        uint32_t &word = P.ent[ent_off + (size_t)(m * W2 + k / 2) * T + th];
+        uint32_t &word = P.ent[ent_off + (size_t)r * W2 + k / 2];
         word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
       }
     }

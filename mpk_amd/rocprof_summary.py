@@ -2,11 +2,14 @@
 
   python -m mpk_amd.rocprof_summary stats <dir>
       per-kernel calls, average duration, VGPRs, LDS and scratch bytes
+  python -m mpk_amd.rocprof_summary counters <dir>
+      average of every collected counter per kernel
   python -m mpk_amd.rocprof_summary pmc <fetch_dir> <write_dir>
       HBM bytes per dispatch of every kernel from a FETCH_SIZE run and a
       WRITE_SIZE run (both counters are KiB per dispatch), each scaled by
-      what the counter reads for the calibration copy kernel k_copy, which
-      reads and writes 8*n bytes (its grid size is n).
+      what the counter reads for the calibration copy kernel k_dcopy
+      (mpkx_dcopy), which reads and writes 8*n bytes (its grid size is n,
+      rounded up to the 256-thread workgroup).
 """
 import glob
 import json
@@ -24,7 +27,8 @@ def _db(d):
 
 
 def short(name):
-    return name.split("(")[0].replace("void ", "")[:100]
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0][:100]
 
 
 def stats(d):
@@ -49,11 +53,11 @@ def _counter(d, name):
 def pmc(fetch_dir, write_dir):
     f = _counter(fetch_dir, "FETCH_SIZE")
     w = _counter(write_dir, "WRITE_SIZE")
-    cal = [k for k in f if "k_copy" in k]
+    cal = [k for k in f if "k_dcopy" in k]
     if not cal:
-        raise RuntimeError("calibration kernel k_copy not found")
+        raise RuntimeError("calibration kernel k_dcopy not found")
     cal = cal[0]
-    grid = f[cal][0][0]
+    grid = max(g for g, _ in f[cal])
     copy_bytes = 8.0 * grid
     fscale = copy_bytes / (sum(v for _, v in f[cal]) / len(f[cal]) * 1024.0)
     wscale = copy_bytes / (sum(v for _, v in w[cal]) / len(w[cal]) * 1024.0)
@@ -67,11 +71,24 @@ def pmc(fetch_dir, write_dir):
     return out
 
 
+def counters(d):
+    """Average value per dispatch of every collected counter, per kernel."""
+    c = _db(d)
+    out = defaultdict(dict)
+    for kname, cname, val in c.execute("select kernel_name, counter_name, avg(value) from counters_collection "
+                                       "group by kernel_name, counter_name"):
+        out[short(kname)][cname] = val
+    return out
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         print(f'{"kernel":70s} {"calls":>6s} {"avg_us":>10s} {"vgpr":>5s} {"lds":>7s} {"scratch":>7s} {"pct":>6s}')
         for r in stats(sys.argv[2]):
             print(f'{r["kernel"][:70]:70s} {r["calls"]:6d} {r["avg_us"]:10.3f} {r["vgpr"]:5d} {r["lds"]:7d} '
                   f'{r["scratch"]:7d} {r["pct"]:6.2f}')
+    elif sys.argv[1] == "counters":
+        for k, v in counters(sys.argv[2]).items():
+            print(k[:70], " ".join(f"{c}={x:.4g}" for c, x in sorted(v.items())))
     elif sys.argv[1] == "pmc":
         print(json.dumps(pmc(sys.argv[2], sys.argv[3]), indent=1))

@@ -216,3 +216,29 @@ def test_fused32_bf16_random_values_stated_error():
         errs = _rel_errs_scaled(to_host(vv), gold, n, k, F.sigma)
         say("values", values, "sigma", F.sigma, "per-power error", errs)
         assert errs.max() <= tol
+
+
+@pytest.mark.parametrize("case", ["m5p64_4_4_2", "cube16_8_8_1", "m9p16_8_6_2", "cube8_8_4_0"])
+def test_task_lists_exec_bitwise_reference(case):
+    """prep_mpk task lists run on the GPU (valued form: vexec_mpk) equal the
+    reference results; the 1/deg form (exec_mpk) equals the oracle's task
+    exec with those weights."""
+    say("tasks", case)
+    d, rp, ci, va, parts, levs = O.case_inputs(case)
+    sk = M.comp_skirt(rp, ci, parts[0], d["npart"], d["nlevel"], levs[-1] if d["nphase"] else None)
+    tcount, tidx = M.prep_mpk(rp, ci, d["nlevel"], d["nphase"], parts, levs, sk)
+    x = O.uniform(d["seed"], d["n"])
+    T = M.TaskPlan(rp, ci, va, d["nlevel"], d["nphase"], d["npart"], tcount, tidx)
+    vv = zeros_dev(d["n"] * (d["nlevel"] + 1))
+    T.exec(to_dev(x), vv)
+    assert O.sha(to_host(vv).tobytes()) == d["results_seeded.bin"]
+    inv = np.repeat(1.0 / np.diff(rp), np.diff(rp))
+    T1 = M.TaskPlan(rp, ci, None, d["nlevel"], d["nphase"], d["npart"], tcount, tidx)
+    vv1 = zeros_dev(d["n"] * (d["nlevel"] + 1))
+    T1.exec(to_dev(x), vv1)
+    ref = O.vexec_tasks(rp, ci, inv, tidx, x, d["nlevel"])
+    assert np.array_equal(to_host(vv1), ref)
+    # exec_mpk's check on x = ones: every power stays 1 (mpk2/mpi2_mpkexec.c:148-157)
+    ones = zeros_dev(d["n"] * (d["nlevel"] + 1))
+    T1.exec(to_dev(np.ones(d["n"])), ones)
+    assert np.allclose(to_host(ones), 1.0, rtol=0, atol=1e-13)
