@@ -274,6 +274,43 @@ int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots,
                      int *launches);
 
 /* ------------------------------------------------------------------ */
+/* pdmpk "partial diamond" schedule (pdmpk/comm_comp_patterns.cc):     */
+/* vertices advance one level per batch, rows may be completed over    */
+/* several batches from partial sums, partitions are redrawn after     */
+/* each phase.  Only the final power is kept (pdmpk's Results).        */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_pd mpkx_pd;
+/* Prep of every partition's buffers (CommCompPatterns ctor + Epilogue,
+ * pdmpk/comm_comp_patterns.cc:26-96).  METIS is replaced by the caller's
+ * partitions: part0 at the start (MetisPartition) and reparts[i %
+ * nrepart] wherever the reference redraws (MetisPartitionWithWeights);
+ * MetisFixVertex and the LAPJV relabelling are kept.  mirror: the
+ * ProcAllPhases0..5 policy (0..5). */
+int mpkx_pd_prep(mpkx_pd **pd, int n, const int *ptr, const int *col,
+                 const double *val, int npart, int nlevel, const int *part0,
+                 int nrepart, const int *const *reparts, int mirror);
+void mpkx_pd_destroy(mpkx_pd *pd);
+int mpkx_pd_info(const mpkx_pd *pd, int *npart, int *nbatch,
+                 long *mbuf_total, long *mcol_total);
+/* Buffers::Dump / Results::Dump layout (pdmpk/buffers.cc:194-206,
+ * pdmpk/results.cc:36-40): <prefix>-r<rank>.buf.bin and
+ * <prefix>-r<rank>.results.bin; Load reads them back. */
+int mpkx_pd_dump(const mpkx_pd *pd, const char *prefix);
+int mpkx_pd_load(mpkx_pd **pd, const char *prefix, int npart, int n,
+                 int nlevel);
+/* GPU exec (Buffers::Exec, pdmpk/buffers.cc:52-97): all partitions' mbufs
+ * in one HBM array, one launch per batch, DoComm as a ghost copy.
+ * LoadInput takes x (device, n); y (device, n) receives the final power
+ * (Results::FillResults).  send_home != 0 also runs SendHome. */
+typedef struct mpkx_pd_plan mpkx_pd_plan;
+int mpkx_pd_plan_create(mpkx_pd_plan **plan, const mpkx_pd *pd);
+void mpkx_pd_plan_destroy(mpkx_pd_plan *plan);
+int mpkx_pd_exec(mpkx_pd_plan *plan, const double *x, double *y,
+                 int send_home, void *stream);
+int mpkx_pd_plan_stats(const mpkx_pd_plan *plan, int *nbatch, long *rows,
+                       long *nnz, long *ghosts);
+
+/* ------------------------------------------------------------------ */
 /* Reduced-precision fused MPK (no reference counterpart: the reference */
 /* is FP64 only).  FP32 vectors and FP32 accumulation; matrix values as */
 /* a <= 4-value dictionary (MPKX_VALUES_AUTO, when the matrix has that  */

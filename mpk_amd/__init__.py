@@ -96,6 +96,15 @@ def lib():
         "mpkx_exec": ([P, P, P, P], C.c_int),
         "mpkx_plan_stats": ([P, P, P, P, P], C.c_int),
         "mpkx_dcopy": ([P, P, C.c_long, P], C.c_int),
+        "mpkx_pd_prep": ([P, C.c_int, P, P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_int], C.c_int),
+        "mpkx_pd_destroy": ([P], None),
+        "mpkx_pd_info": ([P, P, P, P, P], C.c_int),
+        "mpkx_pd_dump": ([P, C.c_char_p], C.c_int),
+        "mpkx_pd_load": ([P, C.c_char_p, C.c_int, C.c_int, C.c_int], C.c_int),
+        "mpkx_pd_plan_create": ([P, P], C.c_int),
+        "mpkx_pd_plan_destroy": ([P], None),
+        "mpkx_pd_exec": ([P, P, P, C.c_int, P], C.c_int),
+        "mpkx_pd_plan_stats": ([P, P, P, P, P], C.c_int),
         "mpkx_tasks_create": ([P, C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, P, P], C.c_int),
         "mpkx_tasks_destroy": ([P], None),
         "mpkx_tasks_exec": ([P, P, P, P], C.c_int),
@@ -451,6 +460,64 @@ class Plan:
     def __del__(self):
         if getattr(self, "_h", None):
             lib().mpkx_plan_destroy(self._h)
+            self._h = None
+
+
+class PdBuffers:
+    """pdmpk partial-diamond buffers of every partition (host)."""
+
+    def __init__(self, handle, n, npart, nlevel):
+        self._h = handle
+        self.n, self.npart, self.nlevel = n, npart, nlevel
+
+    @classmethod
+    def prep(cls, ptr, col, val, npart, nlevel, part0, reparts=(), mirror=0):
+        h = C.c_void_p()
+        reparts = [_i32(r) for r in reparts]
+        arr = (P * max(len(reparts), 1))(*[r.ctypes.data for r in reparts])
+        _check(lib().mpkx_pd_prep(C.byref(h), len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)),
+                                  _p(np.ascontiguousarray(val, np.float64)), npart, nlevel,
+                                  _p(_i32(part0)), len(reparts), C.cast(arr, P), mirror))
+        return cls(h, len(ptr) - 1, npart, nlevel)
+
+    @classmethod
+    def load(cls, prefix, npart, n, nlevel):
+        h = C.c_void_p()
+        _check(lib().mpkx_pd_load(C.byref(h), prefix.encode(), npart, n, nlevel))
+        return cls(h, n, npart, nlevel)
+
+    def dump(self, prefix):
+        _check(lib().mpkx_pd_dump(self._h, prefix.encode()))
+
+    def info(self):
+        npart, nb, mb, mc = C.c_int(), C.c_int(), C.c_long(), C.c_long()
+        _check(lib().mpkx_pd_info(self._h, C.byref(npart), C.byref(nb), C.byref(mb), C.byref(mc)))
+        return {"npart": npart.value, "batches": nb.value, "mbuf": mb.value, "mcol": mc.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_pd_destroy(self._h)
+            self._h = None
+
+
+class PdPlan:
+    """GPU exec of pdmpk buffers; exec(x, y) writes the final power A^k x."""
+
+    def __init__(self, pdbufs):
+        self._h = C.c_void_p()
+        _check(lib().mpkx_pd_plan_create(C.byref(self._h), pdbufs._h))
+
+    def exec(self, x, y, send_home=True, stream=None):
+        _check(lib().mpkx_pd_exec(self._h, _dev(x), _dev(y), int(send_home), _stream(stream)))
+
+    def stats(self):
+        nb, r, z, g = C.c_int(), C.c_long(), C.c_long(), C.c_long()
+        _check(lib().mpkx_pd_plan_stats(self._h, C.byref(nb), C.byref(r), C.byref(z), C.byref(g)))
+        return {"batches": nb.value, "rows": r.value, "nnz": z.value, "ghosts": g.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_pd_plan_destroy(self._h)
             self._h = None
 
 

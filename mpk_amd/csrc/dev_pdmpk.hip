@@ -1,0 +1,265 @@
+// dev_pdmpk.hip -- GPU exec of pdmpk buffers (pdmpk/buffers.cc:52-192).
+//
+// Reference, one MPI rank per partition:
+//   LoadInput; DoComp(0); for b = 1..B-1 { DoComm(b); DoComp(b); } SendHome
+//   DoComp(b): row mi of batch b starts from a partial sum mbuf[init.first]
+//              when the next init pair targets it (else 0.0), then adds
+//              mval[mj] * mbuf[mcol[mj]] in order;
+//   DoComm(b): sbuf = mbuf[sbuf_idcs]; MPI_Alltoallv into the batch's rbuf.
+// Here all partitions' mbufs share one flat HBM array; a batch's rows
+// depend only on earlier batches, so each batch is one launch over all
+// partitions, and each DoComm is one copy kernel over precomputed
+// (destination, source) slot pairs.  Rows are summed in the reference's
+// order with separately rounded products and sums: results are bitwise
+// equal to the reference exec (and to the oracle restatement of it).
+#include <algorithm>
+#include <memory>
+#include <unordered_set>
+
+#include "dev_common.h"
+#include "pdmpk.h"
+
+using namespace mpkx;
+
+namespace {
+
+constexpr int PT = 256;
+constexpr int PCAP = 2048;
+
+// CSR-stream row kernel with an optional partial-sum seed per row.
+__global__ __launch_bounds__(PT) void k_pd_rows(const long *__restrict__ rp, const int *__restrict__ ci,
+                                                const double *__restrict__ va, const int *__restrict__ seed,
+                                                const int *__restrict__ out_row, const int *__restrict__ blk,
+                                                int blk0, double *mb) {
+  __shared__ double prod[PCAP];
+  const int b = blk0 + blockIdx.x;
+  const int r0 = blk[b], r1 = blk[b + 1];
+  const long j0 = rp[r0], j1 = rp[r1];
+  const long len = j1 - j0;
+  if (len <= PCAP) {
+    for (long t = threadIdx.x; t < len; t += PT) prod[t] = __dmul_rn(va[j0 + t], mb[ci[j0 + t]]);
+    __syncthreads();
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += PT) {
+      double s = seed[r] >= 0 ? mb[seed[r]] : 0.0;
+      for (long j = rp[r]; j < rp[r + 1]; j++) s = __dadd_rn(s, prod[j - j0]);
+      mb[out_row[r]] = s;
+    }
+  } else if (threadIdx.x == 0) {
+    double s = seed[r0] >= 0 ? mb[seed[r0]] : 0.0;
+    for (long j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(va[j], mb[ci[j]]));
+    mb[out_row[r0]] = s;
+  }
+}
+
+__global__ void k_pd_copy(double *mb, const int *__restrict__ dst, const int *__restrict__ src, long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) mb[dst[t]] = mb[src[t]];
+}
+
+__global__ void k_pd_gather(double *__restrict__ out, const double *__restrict__ in, const int *__restrict__ idx,
+                            long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[t] = in[idx[t]];
+}
+
+__global__ void k_pd_scatter(double *__restrict__ out, const double *__restrict__ in, const int *__restrict__ idx,
+                             long count) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[idx[t]] = in[t];
+}
+
+inline unsigned g1(long c) { return (unsigned)((c + 255) / 256); }
+
+}  // namespace
+
+struct mpkx_pd_plan {
+  int n = 0, nbatch = 0;
+  long flat = 0;
+  std::vector<int> blk0, nblk;        // per batch
+  std::vector<long> cbegin;           // per batch: range in comm pairs
+  std::vector<char> cstaged;          // per batch
+  long nhome = 0, nsh = 0, nres = 0;
+  DevArray<long> rp;
+  DevArray<int> ci, seed, out_row, blk, cdst, csrc, xdst, xsrc, shdst, shsrc, rdst, rsrc;
+  DevArray<double> va, mb, stage;
+};
+
+extern "C" {
+
+int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
+  clear_error();
+  if (!out || !pd) return fail(MPKX_EINVAL, "pd_plan_create: bad arguments");
+  const auto &B = pd_buffers(pd);
+  const int np = (int)B.size();
+  auto *pl = new (std::nothrow) mpkx_pd_plan;
+  if (!pl) return fail(MPKX_ENOMEM, "pd_plan_create");
+  std::unique_ptr<mpkx_pd_plan> guard(pl);
+  pl->n = pd_n(pd);
+  pl->nbatch = (int)B[0].mbuf_begin.size();
+  std::vector<long> base(np + 1, 0);
+  for (int p = 0; p < np; p++) {
+    if ((int)B[p].mbuf_begin.size() != pl->nbatch || B[p].mptr_begin.size() != (size_t)pl->nbatch + 1 ||
+        B[p].init_begin.size() != (size_t)pl->nbatch + 1 || B[p].sbuf_begin.size() != (size_t)pl->nbatch ||
+        B[p].sendcounts.size() != (size_t)pl->nbatch * np)
+      return fail(MPKX_EINVAL, "pd_plan_create: partition %d has inconsistent batch arrays", p);
+    base[p + 1] = base[p] + B[p].mbuf_idx;
+  }
+  pl->flat = base[np];
+  if (pl->flat >= INT32_MAX) return fail(MPKX_EUNSUPPORTED, "pd_plan_create: mbuf exceeds int32");
+  std::vector<long> rp(1, 0);
+  std::vector<int> ci, seed, outr, blk, cdst, csrc, xdst, xsrc, shdst, shsrc, rdst, rsrc;
+  std::vector<double> va;
+  try {
+    for (int b = 0; b < pl->nbatch; b++) {
+      // DoComm(b): rbuf of batch b follows the batch's computed rows
+      pl->cbegin.push_back((long)cdst.size());
+      if (b > 0) {
+        for (int t = 0; t < np; t++)
+          for (int s = 0; s < np; s++) {
+            int cnt = B[t].recvcounts[(size_t)b * np + s];
+            if (cnt != B[s].sendcounts[(size_t)b * np + t])
+              return fail(MPKX_EINVAL, "pd_plan_create: batch %d count mismatch %d->%d", b, s, t);
+            long rows_t = (long)(B[t].mptr_begin[b + 1] - B[t].mptr_begin[b]);
+            long dbase = base[t] + (long)B[t].mbuf_begin[b] + rows_t + B[t].rdispls[(size_t)b * np + s];
+            long sbase = (long)B[s].sbuf_begin[b] + B[s].sdispls[(size_t)b * np + t];
+            for (int r = 0; r < cnt; r++) {
+              int si = B[s].sbuf_idcs[sbase + r];
+              if (si < 0 || si >= B[s].mbuf_idx) return fail(MPKX_EINVAL, "pd_plan_create: sbuf index out of range");
+              cdst.push_back((int)(dbase + r));
+              csrc.push_back((int)(base[s] + si));
+            }
+          }
+        std::unordered_set<int> dset(cdst.begin() + pl->cbegin[b], cdst.end());
+        bool staged = false;
+        for (size_t q = pl->cbegin[b]; q < csrc.size() && !staged; q++) staged = dset.count(csrc[q]) > 0;
+        pl->cstaged.push_back(staged);
+      } else {
+        pl->cstaged.push_back(0);
+      }
+      // DoComp(b) of every partition (buffers.cc:76-97)
+      int r0 = (int)rp.size() - 1;
+      for (int p = 0; p < np; p++) {
+        const PdBuffers &q = B[p];
+        size_t m0 = q.mptr_begin[b], m1 = q.mptr_begin[b + 1];
+        size_t ii = q.init_begin[b], ie = q.init_begin[b + 1];
+        for (size_t mi = 0; mi < m1 - m0; mi++) {
+          int sd = -1;
+          if (ii < ie && (long)(q.mbuf_begin[b] + mi) == q.init_idcs[ii].second) {
+            sd = (int)(base[p] + q.init_idcs[ii].first);
+            ii++;
+          }
+          int ja = q.mptr[m0 + mi], jb = q.mptr[m0 + mi + 1];
+          for (int j = ja; j < jb; j++) {
+            if (q.mcol[j] < 0 || q.mcol[j] >= q.mbuf_idx) return fail(MPKX_EINVAL, "pd_plan_create: mcol out of range");
+            ci.push_back((int)(base[p] + q.mcol[j]));
+            va.push_back(q.mval[j]);
+          }
+          rp.push_back((long)ci.size());
+          seed.push_back(sd);
+          outr.push_back((int)(base[p] + q.mbuf_begin[b] + mi));
+        }
+      }
+      int r1 = (int)rp.size() - 1;
+      pl->blk0.push_back((int)blk.size());
+      int r = r0;
+      while (r < r1) {  // workgroups of consecutive rows whose entries fit PCAP
+        blk.push_back(r);
+        long start = rp[r];
+        int e = r + 1;
+        if (rp[e] - start <= PCAP)
+          while (e < r1 && rp[e + 1] - start <= PCAP && e - r < 4 * PT) e++;
+        r = e;
+      }
+      pl->nblk.push_back((int)blk.size() - pl->blk0.back());
+    }
+    pl->cbegin.push_back((long)cdst.size());
+    blk.push_back((int)rp.size() - 1);
+    for (int p = 0; p < np; p++) {
+      const PdBuffers &q = B[p];
+      long nhome = (long)q.mbuf_begin[0];  // LoadInput: the home vertices' level 0
+      if (nhome > (long)q.original_idcs.size()) return fail(MPKX_EINVAL, "pd_plan_create: original_idcs short");
+      for (long i = 0; i < nhome; i++) {
+        xdst.push_back((int)(base[p] + i));
+        xsrc.push_back(q.original_idcs[i]);
+      }
+      for (size_t i = 0; i < q.home_idcs.size(); i++) {  // SendHome
+        shdst.push_back((int)(base[p] + i));
+        shsrc.push_back((int)(base[p] + q.home_idcs[i]));
+      }
+      for (size_t i = 0; i < q.res_mbuf_idcs.size(); i++) {  // Results::FillVal / FillResults
+        rdst.push_back(q.res_original_idcs[i]);
+        rsrc.push_back((int)(base[p] + q.res_mbuf_idcs[i]));
+      }
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(MPKX_ENOMEM, "pd_plan_create: host allocation");
+  }
+  pl->nhome = (long)xdst.size();
+  pl->nsh = (long)shdst.size();
+  pl->nres = (long)rdst.size();
+  long stage_n = 1;
+  for (size_t b = 0; b + 1 < pl->cbegin.size(); b++) stage_n = std::max(stage_n, pl->cbegin[b + 1] - pl->cbegin[b]);
+  stage_n = std::max(stage_n, pl->nsh);
+  int rc;
+  if ((rc = pl->rp.upload(rp)) || (rc = pl->ci.upload(ci)) || (rc = pl->va.upload(va)) ||
+      (rc = pl->seed.upload(seed)) || (rc = pl->out_row.upload(outr)) || (rc = pl->blk.upload(blk)) ||
+      (rc = pl->cdst.upload(cdst)) || (rc = pl->csrc.upload(csrc)) || (rc = pl->xdst.upload(xdst)) ||
+      (rc = pl->xsrc.upload(xsrc)) || (rc = pl->shdst.upload(shdst)) || (rc = pl->shsrc.upload(shsrc)) ||
+      (rc = pl->rdst.upload(rdst)) || (rc = pl->rsrc.upload(rsrc)) || (rc = pl->mb.alloc(pl->flat)) ||
+      (rc = pl->stage.alloc(stage_n)))
+    return rc;
+  *out = guard.release();
+  return MPKX_OK;
+}
+
+void mpkx_pd_plan_destroy(mpkx_pd_plan *pl) { delete pl; }
+
+int mpkx_pd_exec(mpkx_pd_plan *pl, const double *x, double *y, int send_home, void *stream) {
+  clear_error();
+  if (!pl || !x || !y) return fail(MPKX_EINVAL, "pd_exec: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  // LoadInput: mbuf[i] = x[original_idcs[i]] (the reference loads ones)
+  if (pl->nhome > 0) {
+    hipLaunchKernelGGL(k_pd_gather, dim3(g1(pl->nhome)), dim3(256), 0, st, pl->stage.p, x, pl->xsrc.p, pl->nhome);
+    hipLaunchKernelGGL(k_pd_scatter, dim3(g1(pl->nhome)), dim3(256), 0, st, pl->mb.p, pl->stage.p, pl->xdst.p,
+                       pl->nhome);
+  }
+  for (int b = 0; b < pl->nbatch; b++) {
+    long c0 = pl->cbegin[b], cnt = pl->cbegin[b + 1] - c0;
+    if (cnt > 0) {
+      if (!pl->cstaged[b]) {
+        hipLaunchKernelGGL(k_pd_copy, dim3(g1(cnt)), dim3(256), 0, st, pl->mb.p, pl->cdst.p + c0, pl->csrc.p + c0, cnt);
+      } else {
+        hipLaunchKernelGGL(k_pd_gather, dim3(g1(cnt)), dim3(256), 0, st, pl->stage.p, pl->mb.p, pl->csrc.p + c0, cnt);
+        hipLaunchKernelGGL(k_pd_scatter, dim3(g1(cnt)), dim3(256), 0, st, pl->mb.p, pl->stage.p, pl->cdst.p + c0, cnt);
+      }
+    }
+    if (pl->nblk[b] > 0)
+      hipLaunchKernelGGL(k_pd_rows, dim3(pl->nblk[b]), dim3(PT), 0, st, pl->rp.p, pl->ci.p, pl->va.p, pl->seed.p,
+                         pl->out_row.p, pl->blk.p, pl->blk0[b], pl->mb.p);
+  }
+  if (pl->nres > 0)  // results are read before SendHome moves values
+    hipLaunchKernelGGL(k_pd_gather, dim3(g1(pl->nres)), dim3(256), 0, st, pl->stage.p, pl->mb.p, pl->rsrc.p,
+                       pl->nres);
+  if (pl->nres > 0)
+    hipLaunchKernelGGL(k_pd_scatter, dim3(g1(pl->nres)), dim3(256), 0, st, y, pl->stage.p, pl->rdst.p, pl->nres);
+  if (send_home && pl->nsh > 0) {  // SendHome: mbuf[i] = mbuf[home_idcs[i]]
+    hipLaunchKernelGGL(k_pd_gather, dim3(g1(pl->nsh)), dim3(256), 0, st, pl->stage.p, pl->mb.p, pl->shsrc.p, pl->nsh);
+    hipLaunchKernelGGL(k_pd_scatter, dim3(g1(pl->nsh)), dim3(256), 0, st, pl->mb.p, pl->stage.p, pl->shdst.p,
+                       pl->nsh);
+  }
+  MPKX_HIP(hipGetLastError());
+  return MPKX_OK;
+}
+
+int mpkx_pd_plan_stats(const mpkx_pd_plan *pl, int *nbatch, long *rows, long *nnz, long *ghosts) {
+  clear_error();
+  if (!pl) return fail(MPKX_EINVAL, "pd_plan_stats: null");
+  if (nbatch) *nbatch = pl->nbatch;
+  if (rows) *rows = (long)pl->out_row.count;
+  if (nnz) *nnz = (long)pl->ci.count;
+  if (ghosts) *ghosts = (long)pl->cdst.count;
+  return MPKX_OK;
+}
+
+}  // extern "C"

@@ -24,6 +24,10 @@
  *   orc_exec_bufs       bitwise equal to reference mpi2_mpkexecbufs_val
  *                       results (harness with seeded x) and to orc_spmv_k
  *   orc_spmv_k          bitwise equal to reference verify_val gold_result.bin
+ *   orc_pd_exec         the final power equals orc_spmv_k to <= 1e-13
+ *                       normwise (partial sums reorder the additions)
+ *                       and passes the reference cosine check
+ *                       (apps/check.cc:75-83)
  *   orc_prep_tasks /
  *   orc_vexec_tasks     bitwise equal to orc_spmv_k (shared-memory path has
  *                       no reference output file to compare with: the
@@ -916,4 +920,133 @@ ORC_API void orc_vexec_tasks(int n, const int *ptr, const int *col,
       s += val[j] * vv[(level - 1) * n + col[j]];
     vv[level * n + i] = s;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* pdmpk exec (pdmpk/buffers.cc:52-192, mpi_buffers.cc, results.cc):  */
+/* Buffers::Load of <prefix>-r<rank>.buf.bin, LoadInput (x instead of */
+/* ones), DoComp(0); for b >= 1: DoComm(b) (MPI_Alltoallv restated as */
+/* copies between the ranks' send buffers and receive slots),          */
+/* DoComp(b); then Results::FillVal/FillResults -> y (final power).   */
+/* File layout: Utils::DumpVec = size_t count + raw (utils.hpp:19-24). */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  size_t max_sbuf;
+  int mbuf_n;
+  size_t *mbuf_begin, nb;
+  int *home, *orig;
+  size_t nhome, norig;
+  int *res_mbuf;
+  size_t nres;
+  int *sc, *sd, *rc, *rd;
+  int *sbuf_idcs;
+  size_t *sbuf_begin;
+  int *init;
+  size_t ninit, *init_begin;
+  int *mptr;
+  size_t *mptr_begin;
+  int *mcol;
+  double *mval;
+  int *res_orig;
+  double *mb, *sbuf;
+} orc_pd;
+
+static void *pd_vec(FILE *f, size_t elem, size_t *count) {
+  size_t s = 0;
+  if (fread(&s, sizeof s, 1, f) != 1) return NULL;
+  void *p = malloc(elem * (s ? s : 1));
+  if (s && fread(p, elem, s, f) != s) { free(p); return NULL; }
+  if (count) *count = s;
+  return p;
+}
+
+static int pd_load(const char *prefix, int r, orc_pd *b) {
+  char fn[4096];
+  snprintf(fn, sizeof fn, "%s-r%d.buf.bin", prefix, r);
+  FILE *f = fopen(fn, "rb");
+  if (!f) return -1;
+  size_t dummy, pds_n;
+  if (fread(&b->max_sbuf, sizeof(size_t), 1, f) != 1 || fread(&b->mbuf_n, 4, 1, f) != 1) { fclose(f); return -1; }
+  b->mbuf_begin = pd_vec(f, sizeof(size_t), &b->nb);
+  b->home = pd_vec(f, 4, &b->nhome);
+  b->orig = pd_vec(f, 4, &b->norig);
+  free(pd_vec(f, 6, &pds_n)); /* phase descriptors: 3 shorts each */
+  b->res_mbuf = pd_vec(f, 4, &b->nres);
+  b->sc = pd_vec(f, 4, &dummy);
+  b->sd = pd_vec(f, 4, &dummy);
+  b->rc = pd_vec(f, 4, &dummy);
+  b->rd = pd_vec(f, 4, &dummy);
+  b->sbuf_idcs = pd_vec(f, 4, &dummy);
+  b->sbuf_begin = pd_vec(f, sizeof(size_t), &dummy);
+  b->init = pd_vec(f, 8, &b->ninit);
+  b->init_begin = pd_vec(f, sizeof(size_t), &dummy);
+  b->mptr = pd_vec(f, 4, &dummy);
+  b->mptr_begin = pd_vec(f, sizeof(size_t), &dummy);
+  b->mcol = pd_vec(f, 4, &dummy);
+  b->mval = pd_vec(f, 8, &dummy);
+  fclose(f);
+  snprintf(fn, sizeof fn, "%s-r%d.results.bin", prefix, r);
+  f = fopen(fn, "rb");
+  if (!f) return -1;
+  b->res_orig = pd_vec(f, 4, &dummy);
+  free(pd_vec(f, 8, &dummy));
+  fclose(f);
+  b->mb = calloc(b->mbuf_n ? b->mbuf_n : 1, sizeof(double));
+  b->sbuf = calloc(b->max_sbuf ? b->max_sbuf : 1, sizeof(double));
+  return b->mval ? 0 : -1;
+}
+
+static void pd_comp(orc_pd *b, size_t ph) { /* Buffers::DoComp, buffers.cc:76-97 */
+  size_t mcount = b->mptr_begin[ph + 1] - b->mptr_begin[ph];
+  double *cur = b->mb + b->mbuf_begin[ph];
+  const int *mp = b->mptr + b->mptr_begin[ph];
+  size_t ii = b->init_begin[ph];
+  for (size_t mi = 0; mi < mcount; mi++) {
+    double t = 0.0;
+    if (b->ninit > 0 && ii < b->init_begin[ph + 1] &&
+        b->mbuf_begin[ph] + mi == (size_t)b->init[2 * ii + 1]) {
+      t = b->mb[b->init[2 * ii]];
+      ii++;
+    }
+    for (int j = mp[mi]; j < mp[mi + 1]; j++) t += b->mval[j] * b->mb[b->mcol[j]];
+    cur[mi] = t;
+  }
+}
+
+ORC_API int orc_pd_exec(const char *prefix, int np, const double *x, double *y) {
+  orc_pd *b = calloc(np, sizeof(orc_pd));
+  for (int r = 0; r < np; r++)
+    if (pd_load(prefix, r, &b[r])) return -1;
+  for (int r = 0; r < np; r++) /* LoadInput */
+    for (size_t i = 0; i < b[r].mbuf_begin[0]; i++) b[r].mb[i] = x[b[r].orig[i]];
+  size_t nb = b[0].nb;
+  for (int r = 0; r < np; r++) pd_comp(&b[r], 0);
+  for (size_t ph = 1; ph < nb; ph++) {
+    for (int r = 0; r < np; r++) { /* fill_sbuf */
+      size_t idx = (size_t)np * ph + np - 1;
+      size_t sz = (size_t)b[r].sc[idx] + b[r].sd[idx];
+      const int *si = b[r].sbuf_idcs + b[r].sbuf_begin[ph];
+      for (size_t i = 0; i < sz; i++) b[r].sbuf[i] = b[r].mb[si[i]];
+    }
+    for (int t = 0; t < np; t++)
+      for (int s = 0; s < np; s++) { /* MPI_Alltoallv */
+        size_t o = (size_t)np * ph;
+        int cnt = b[t].rc[o + s];
+        size_t rows = b[t].mptr_begin[ph + 1] - b[t].mptr_begin[ph];
+        double *rbuf = b[t].mb + b[t].mbuf_begin[ph] + rows + b[t].rd[o + s];
+        memcpy(rbuf, b[s].sbuf + b[s].sd[o + t], sizeof(double) * cnt);
+      }
+    for (int r = 0; r < np; r++) pd_comp(&b[r], ph);
+  }
+  for (int r = 0; r < np; r++) /* Results::FillVal + FillResults */
+    for (size_t i = 0; i < b[r].nres; i++) y[b[r].res_orig[i]] = b[r].mb[b[r].res_mbuf[i]];
+  for (int r = 0; r < np; r++) {
+    free(b[r].mbuf_begin); free(b[r].home); free(b[r].orig); free(b[r].res_mbuf);
+    free(b[r].sc); free(b[r].sd); free(b[r].rc); free(b[r].rd); free(b[r].sbuf_idcs);
+    free(b[r].sbuf_begin); free(b[r].init); free(b[r].init_begin); free(b[r].mptr);
+    free(b[r].mptr_begin); free(b[r].mcol); free(b[r].mval); free(b[r].res_orig);
+    free(b[r].mb); free(b[r].sbuf);
+  }
+  free(b);
+  return 0;
 }

@@ -41,6 +41,8 @@ def lib():
         L.orc_prep_tasks.argtypes = [C.c_int, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P, P]
         L.orc_prep_tasks.restype = C.c_long
         L.orc_vexec_tasks.argtypes = [C.c_int, P, P, P, C.c_long, P, P]
+        L.orc_pd_exec.argtypes = [C.c_char_p, C.c_int, P, P]
+        L.orc_pd_exec.restype = C.c_int
         _lib = L
     return _lib
 
@@ -158,6 +160,15 @@ def vexec_tasks(rp, ci, va, tidx, x, nlevel):
     vv[:n] = x
     lib().orc_vexec_tasks(n, ptr(rp), ptr(ci), ptr(va), len(tidx), ptr(tidx), ptr(vv))
     return vv
+
+
+def pd_exec(prefix, npart, x):
+    """pdmpk exec of dumped buffers; returns the final power."""
+    y = np.zeros(len(x))
+    rc = lib().orc_pd_exec(prefix.encode(), npart, ptr(np.ascontiguousarray(x)), ptr(y))
+    if rc != 0:
+        raise RuntimeError("oracle could not read the pdmpk buffers")
+    return y
 
 
 # ---- golden fixture helpers -------------------------------------------------

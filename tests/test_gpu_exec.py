@@ -242,3 +242,46 @@ def test_task_lists_exec_bitwise_reference(case):
     ones = zeros_dev(d["n"] * (d["nlevel"] + 1))
     T1.exec(to_dev(np.ones(d["n"])), ones)
     assert np.allclose(to_host(ones), 1.0, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("ci_case", [0, 3, 4, 5])
+def test_pdmpk_gpu_bitwise_oracle(ci_case, tmp_path):
+    """pdmpk buffers executed on the GPU equal the oracle exec bitwise (same
+    partial-sum order) and k x SpMV to 1e-13."""
+    from tests.test_pdmpk import CASES, pd_case
+    case = CASES[ci_case]
+    say("pdmpk", case)
+    rp, ci, va, pd = pd_case(*case)
+    n, npart, nlevel = len(rp) - 1, case[4], case[5]
+    prefix = str(tmp_path / "pd")
+    pd.dump(prefix)
+    x = M.uniform_pm1(0x5EED, n)
+    ref = O.pd_exec(prefix, npart, x)
+    plan = M.PdPlan(pd)
+    y = zeros_dev(n)
+    plan.exec(to_dev(x), y)
+    out = to_host(y)
+    assert np.array_equal(out, ref)
+    gold = O.spmv_k(rp, ci, va, x, nlevel)[nlevel * n:]
+    assert np.linalg.norm(out - gold) / np.linalg.norm(gold) <= 1e-13
+    # buffers reloaded from the files run the same
+    plan2 = M.PdPlan(M.PdBuffers.load(prefix, npart, n, nlevel))
+    y2 = zeros_dev(n)
+    plan2.exec(to_dev(x), y2)
+    assert np.array_equal(to_host(y2), ref)
+
+
+def test_pdmpk_gpu_cube32():
+    say("pdmpk cube32")
+    N, k = 32, 6
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    n = len(rp) - 1
+    part0 = M.partition_grid([N] * 3, [2, 2, 2])
+    reps = [M.partition_grid([N] * 3, [2, 2, 2], [8] * 3)]
+    pd = M.PdBuffers.prep(rp, ci, va, 8, k, part0, reps, 0)
+    x = M.uniform_pm1(3, n)
+    gold = O.spmv_k(rp, ci, va, x, k)[k * n:]
+    y = zeros_dev(n)
+    M.PdPlan(pd).exec(to_dev(x), y)
+    out = to_host(y)
+    assert np.linalg.norm(out - gold) / np.linalg.norm(gold) <= 1e-13
