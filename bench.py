@@ -125,6 +125,88 @@ def measured_traffic(N, k, s, blocks):
     return None
 
 
+def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C):
+    """256 Chung-Lu matrices (rows log-uniform in [1k, 200k], exponent 2.5,
+    about 10 entries per row, N(0,1) values, diagonal = row abs-sum + 1), FP32,
+    8 right-hand sides each, k = 6, all powers of all matrices in one hipGraph
+    launch; configuration picked by the exhaustive autotuner.
+    Bytes per power (the roofline unit): 8*nnz + 4*(rows+1) + 2*4*rows*nrhs."""
+    from concurrent.futures import ThreadPoolExecutor
+    from mpk_amd import costmodel as CM
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(1000), np.log(200000), count)).astype(int)
+    seeds = rng.integers(1, 2 ** 62, count)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(16) as pool:
+        mats = list(pool.map(lambda i: M.gen_chunglu(int(ns[i]), 10.0, 2.5, 1.0, int(seeds[i])), range(count)))
+    t_gen = time.perf_counter() - t0
+    bm = M.BatchMPK(mats, nrhs, k)
+    rows, nnz = bm.rows, bm.stats()["nnz"]
+    X = torch.rand(rows * nrhs, device=dev, dtype=torch.float32) * 2 - 1
+    VV = torch.empty(rows * nrhs * (k + 1), device=dev, dtype=torch.float32)
+    t0 = time.perf_counter()
+    times = CM.tune(bm, X, VV, reps=3)
+    t_tune = time.perf_counter() - t0
+    best = min(times, key=times.get)
+    bm.configure(*best)
+    ms = CM.time_exec(bm, X, VV, reps=steps, warmup=2)
+    # error of the smallest matrix against FP64 (scipy) per power, one right-hand side
+    m0 = int(np.argmin(ns))
+    rp, ci, va = mats[m0]
+    import scipy.sparse as sp
+    A = sp.csr_matrix((va / bm.sigmas[m0], ci, rp))
+    r0, r1 = int(bm.row_offsets[m0]), int(bm.row_offsets[m0 + 1])
+    got = VV.view(k + 1, rows, nrhs)[:, r0:r1, 0].double().cpu().numpy()
+    ref = got[0].copy()
+    errs = []
+    for l in range(1, k + 1):
+        ref = A @ ref
+        errs.append(float("%.3e" % (np.linalg.norm(got[l] - ref) / np.linalg.norm(ref))))
+    bpp = 8 * nnz + 4 * (rows + 1) + 2 * 4 * rows * nrhs
+    gbs = k * bpp / (ms * 1e-3) / 1e9
+    del mats
+    return {"workload": f"{count} Chung-Lu matrices, rows {rows}, nnz {nnz}, {nrhs} RHS, k={k}, FP32",
+            "ms_per_step": round(ms, 5), "value": round(gbs, 2), "unit": "GB/s",
+            "pct_hbm_roofline": round(100 * gbs / HBM_PEAK_GBS, 2),
+            "bytes_per_power": bpp, "config": list(best), "default_config_ms": round(times[CM.default_config(nrhs)], 5),
+            "configs_tried": len(times), "tune_seconds": round(t_tune, 2), "generate_seconds": round(t_gen, 2),
+            "launches_in_graph": bm.stats()["launches"], "long_rows": bm.stats()["long_rows"],
+            "per_power_rel_err_vs_fp64_smallest_matrix": errs}
+
+
+def learned_tuning(M, torch, dev, path, n=1500000, nrhs=16, k=10, seed=0xC5):
+    """Held-out 1.5M-row Chung-Lu matrix (about 12 entries per row), 16 RHS,
+    k = 10: the cost model's pick against the exhaustive autotuner's."""
+    from mpk_amd import costmodel as CM
+    if not os.path.exists(path):
+        return {"error": "no trained cost model"}
+    model = CM.CostModel.load(path)
+    rp, ci, va = M.gen_chunglu(n, 12.0, 2.5, 1.0, seed)
+    t0 = time.perf_counter()
+    f = M.features(rp, ci, nrhs)
+    pred = model.predict(f)[0]
+    pick = CM.nearest(pred, nrhs)
+    t_model = time.perf_counter() - t0
+    bm = M.BatchMPK([(rp, ci, va)], nrhs, k)
+    X = torch.rand(n * nrhs, device=dev, dtype=torch.float32) * 2 - 1
+    VV = torch.empty(n * nrhs * (k + 1), device=dev, dtype=torch.float32)
+    t0 = time.perf_counter()
+    times = CM.tune(bm, X, VV, reps=3)
+    t_sweep = time.perf_counter() - t0
+    best = min(times, key=times.get)
+    dflt = CM.default_config(nrhs)
+    rank = sorted(times.values()).index(times[pick]) + 1
+    return {"workload": f"Chung-Lu n={n}, nnz={int(rp[-1])}, {nrhs} RHS, k={k}, FP32",
+            "model_pick": list(pick), "model_ms": round(times[pick], 5),
+            "exhaustive_pick": list(best), "exhaustive_ms": round(times[best], 5),
+            "default_config": list(dflt), "default_ms": round(times[dflt], 5),
+            "model_regret": round(times[pick] / times[best], 4),
+            "default_regret": round(times[dflt] / times[best], 4),
+            "model_rank": f"{rank}/{len(times)}", "in_training_range": model.in_range(f),
+            "model_seconds": round(t_model, 4), "exhaustive_seconds": round(t_sweep, 2),
+            "trained_on": model.meta.get("samples")}
+
+
 def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
     """k back-to-back SpMVs through torch.sparse CSR (hipSPARSE/rocSPARSE on
     ROCm), as an outside reference point for the plain kernel."""
@@ -165,6 +247,8 @@ def main():
     ap.add_argument("--cold-reps", type=int, default=10, help="cold-MALL repetitions (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-matrix form")
+    ap.add_argument("--no-tuning", action="store_true", help="skip the learned-tuning evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -330,8 +414,22 @@ def main():
                      "kernel": "k_fused (one round of s levels)",
                      "round_ms": [round(t, 5) for t in round_ms]},
     }
+    host_vv = vf.cpu().numpy()
+    del F, A, vf, vp
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        try:
+            out["batched_form"] = batched_form(M, torch, dev)
+        except Exception as e:
+            out["batched_form"] = {"error": str(e)}
+        torch.cuda.empty_cache()
+    if not args.no_tuning:
+        try:
+            out["learned_tuning"] = learned_tuning(M, torch, dev, os.path.join(REPO, "mpk_amd", "cost_model.json"))
+        except Exception as e:
+            out["learned_tuning"] = {"error": str(e)}
+        torch.cuda.empty_cache()
     if not args.no_cpu:
-        host_vv = vf.cpu().numpy()
         cb = None
         try:
             cb = cpu_baseline_reference(M, rp, ci, va, x, N, k, lambda r: np.array_equal(r, host_vv))

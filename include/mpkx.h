@@ -341,6 +341,77 @@ int mpkx_norms32(const float *vv, int n, int nlevel, double *norms,
 int mpkx_residual32(mpkx_csr *A, const float *vv, int level, double sigma,
                     double *residual, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Synthetic matrices for the batched and tuning workloads (no         */
+/* reference counterpart; the reference generates stencils only,       */
+/* mpk2/gen2.c).  Both return malloc'ed CSR arrays (caller frees), rows */
+/* sorted by column, N(0,1) off-diagonals from SplitMix64 + Box-Muller,*/
+/* diagonal = row abs-sum + diag_shift.                                */
+/* ------------------------------------------------------------------ */
+/* Chung-Lu power law: expected degree ~ (i+1)^(-1/(exponent-1)),      */
+/* (mean_nnz - 3) n / 2 sampled edges, symmetric, plus a ring.          */
+int mpkx_gen_chunglu(int n, double mean_nnz, double exponent,
+                     double diag_shift, uint64_t seed, int **ptr, int **col,
+                     double **val);
+/* Banded: per_row offsets spread evenly over [-bandwidth, bandwidth]. */
+int mpkx_gen_banded(int n, int bandwidth, int per_row, double diag_shift,
+                    uint64_t seed, int **ptr, int **col, double **val);
+
+/* ------------------------------------------------------------------ */
+/* Batched block-RHS MPK (no reference counterpart).  nmat matrices are */
+/* concatenated block-diagonally, values FP32 scaled by 1/sigma_m (the  */
+/* power of two at or above matrix m's largest row abs-sum; sigmas[m]   */
+/* returned when non-NULL).  Vectors FP32 row-major [row][nrhs] over    */
+/* all rows of all matrices; VV level-major:                            */
+/*   VV[(l*rows + r)*nrhs + c] = ((A_m/sigma_m)^l X)[r][c].             */
+/* All powers run as one hipGraph launch.  nrhs: power of two <= 64.    */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_batch mpkx_batch;
+int mpkx_batch_create(mpkx_batch **b, int nmat, const int *n,
+                      const int *const *ptr, const int *const *col,
+                      const double *const *val, int nrhs, int nlevel,
+                      double *sigmas);
+void mpkx_batch_destroy(mpkx_batch *b);
+/* Tuning configuration: block_width right-hand sides per pass (power of */
+/* two <= 16 dividing nrhs), lanes_per_row lanes splitting each row's    */
+/* entries (power of two, width*lanes <= 64), rows with more than        */
+/* long_row entries get a workgroup each.  Default (min(nrhs,8), 1, 256).*/
+int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row,
+                         int long_row);
+/* X: device FP32 rows*nrhs; VV: device FP32 rows*nrhs*(nlevel+1) (may  */
+/* begin at X, then the copy of level 0 is skipped).                    */
+int mpkx_batch_exec(mpkx_batch *b, const float *X, float *VV, void *stream,
+                    int use_graph);
+int mpkx_batch_stats(const mpkx_batch *b, long *rows, long *nnz,
+                     long *long_rows, int *launches);
+
+/* ------------------------------------------------------------------ */
+/* Learned tuning (no reference counterpart).                          */
+/* mpkx_features: 24 numbers describing a CSR matrix and a right-hand- */
+/* side count: log sizes, row-length distribution (mean, CV, max,      */
+/* quantiles, long-row fractions), column spread (mean/max |i-j|,      */
+/* locality), skirt growth |ball_d(S)|/|S| of row blocks for d = 1..4, */
+/* log2 nrhs, vector and matrix bytes.                                  */
+/* ------------------------------------------------------------------ */
+int mpkx_features(int n, const int *ptr, const int *col, int nrhs,
+                  double *feat);
+/* Cost model: a 24 -> 64 -> 64 -> 3 MLP (ReLU, linear heads) trained   */
+/* and evaluated on the GPU.  Parameters (FP32, row-major out x in):    */
+/*   W1[64][24] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]                */
+/* loss = 1/(2N) sum (y - t)^2.  X, Y, T are device arrays.             */
+typedef struct mpkx_mlp mpkx_mlp;
+long mpkx_mlp_nparam(void);
+int mpkx_mlp_create(mpkx_mlp **m, const float *params);
+void mpkx_mlp_destroy(mpkx_mlp *m);
+int mpkx_mlp_set_params(mpkx_mlp *m, const float *params);
+int mpkx_mlp_get(mpkx_mlp *m, float *params, float *grad);
+int mpkx_mlp_forward(mpkx_mlp *m, int N, const float *X, float *Y,
+                     void *stream);
+int mpkx_mlp_loss_grad(mpkx_mlp *m, int N, const float *X, const float *T,
+                       float *loss, void *stream);
+int mpkx_mlp_adam(mpkx_mlp *m, float lr, float beta1, float beta2, float eps,
+                  float weight_decay, void *stream);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

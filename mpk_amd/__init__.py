@@ -123,6 +123,22 @@ def lib():
         "mpkx_fused32_stats": ([P, P, P, P, P, P, P], C.c_int),
         "mpkx_norms32": ([P, C.c_int, C.c_int, P, P], C.c_int),
         "mpkx_residual32": ([P, P, C.c_int, C.c_double, P, P], C.c_int),
+        "mpkx_gen_chunglu": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, P, P, P], C.c_int),
+        "mpkx_gen_banded": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, P, P, P], C.c_int),
+        "mpkx_batch_create": ([P, C.c_int, P, P, P, P, C.c_int, C.c_int, P], C.c_int),
+        "mpkx_batch_destroy": ([P], None),
+        "mpkx_batch_configure": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
+        "mpkx_batch_exec": ([P, P, P, P, C.c_int], C.c_int),
+        "mpkx_batch_stats": ([P, P, P, P, P], C.c_int),
+        "mpkx_features": ([C.c_int, P, P, C.c_int, P], C.c_int),
+        "mpkx_mlp_nparam": ([], C.c_long),
+        "mpkx_mlp_create": ([P, P], C.c_int),
+        "mpkx_mlp_destroy": ([P], None),
+        "mpkx_mlp_set_params": ([P, P], C.c_int),
+        "mpkx_mlp_get": ([P, P, P], C.c_int),
+        "mpkx_mlp_forward": ([P, C.c_int, P, P, P], C.c_int),
+        "mpkx_mlp_loss_grad": ([P, C.c_int, P, P, P, P], C.c_int),
+        "mpkx_mlp_adam": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -671,3 +687,130 @@ def residual32(csr, vv, level, sigma, stream=None):
     r = C.c_double()
     _check(lib().mpkx_residual32(csr._h, _dev(vv), level, sigma, C.byref(r), _stream(stream)))
     return r.value
+
+
+# ---- synthetic matrices and the batched block-RHS form -----------------------
+
+def _take_csr(n, rp_p, ci_p, va_p):
+    rp = _take(rp_p, n + 1, np.int32)
+    nnz = int(rp[-1])
+    return rp, _take(ci_p, nnz, np.int32), _take(va_p, nnz, np.float64)
+
+
+def gen_chunglu(n, mean_nnz=10.0, exponent=2.5, diag_shift=1.0, seed=1):
+    """Chung-Lu power-law matrix (about mean_nnz entries per row), N(0,1)
+    off-diagonals, diagonal = row abs-sum + diag_shift."""
+    rp, ci, va = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _check(lib().mpkx_gen_chunglu(n, mean_nnz, exponent, diag_shift, seed, C.byref(rp), C.byref(ci),
+                                  C.byref(va)))
+    return _take_csr(n, rp, ci, va)
+
+
+def gen_banded(n, bandwidth, per_row, diag_shift=1.0, seed=1):
+    """Banded matrix, per_row offsets spread over [-bandwidth, bandwidth]."""
+    rp, ci, va = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _check(lib().mpkx_gen_banded(n, bandwidth, per_row, diag_shift, seed, C.byref(rp), C.byref(ci),
+                                 C.byref(va)))
+    return _take_csr(n, rp, ci, va)
+
+
+class BatchMPK:
+    """Batched block-RHS MPK over many matrices in one hipGraph launch.
+
+    VV (device FP32, (nlevel+1) * rows * nrhs) holds
+    VV[l][r][c] = ((A_m / sigma_m)^l X)[r][c] for row r of matrix m, rows of
+    all matrices concatenated in order (offsets in self.row_offsets)."""
+
+    def __init__(self, mats, nrhs, nlevel):
+        self._h = C.c_void_p()
+        self.nmat = len(mats)
+        self.nrhs = nrhs
+        self.nlevel = nlevel
+        keep = [(_i32(p), _i32(c), np.ascontiguousarray(v, np.float64)) for p, c, v in mats]
+        ns = np.array([len(p) - 1 for p, _, _ in keep], np.int32)
+        pa = (C.c_void_p * self.nmat)(*[_p(p).value for p, _, _ in keep])
+        ca = (C.c_void_p * self.nmat)(*[_p(c).value for _, c, _ in keep])
+        va = (C.c_void_p * self.nmat)(*[_p(v).value for _, _, v in keep])
+        self.sigmas = np.zeros(self.nmat)
+        _check(lib().mpkx_batch_create(C.byref(self._h), self.nmat, _p(ns), pa, ca, va, nrhs, nlevel,
+                                       _p(self.sigmas)))
+        self.row_offsets = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
+        self.rows = int(self.row_offsets[-1])
+        self.config = (min(nrhs, 8), 1, 256)
+
+    def configure(self, block_width, lanes_per_row, long_row):
+        _check(lib().mpkx_batch_configure(self._h, block_width, lanes_per_row, long_row))
+        self.config = (block_width, lanes_per_row, long_row)
+
+    def exec(self, X, VV, stream=None, use_graph=True):
+        _check(lib().mpkx_batch_exec(self._h, _dev(X), _dev(VV), _stream(stream), int(use_graph)))
+
+    def stats(self):
+        r, z, lr, la = C.c_long(), C.c_long(), C.c_long(), C.c_int()
+        _check(lib().mpkx_batch_stats(self._h, C.byref(r), C.byref(z), C.byref(lr), C.byref(la)))
+        return {"rows": r.value, "nnz": z.value, "long_rows": lr.value, "launches": la.value,
+                "config": self.config}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_batch_destroy(self._h)
+            self._h = None
+
+
+# ---- learned tuning ------------------------------------------------------------
+
+NFEAT = 24
+
+
+def features(ptr, col, nrhs):
+    """The 24 matrix/workload features the cost model reads."""
+    ptr = _i32(ptr)
+    out = np.zeros(NFEAT)
+    _check(lib().mpkx_features(len(ptr) - 1, _p(ptr), _p(_i32(col)), nrhs, _p(out)))
+    return out
+
+
+class Mlp:
+    """24 -> 64 -> 64 -> 3 MLP on the GPU (forward, loss/gradient, Adam)."""
+
+    def __init__(self, params):
+        self._h = C.c_void_p()
+        self.nparam = int(lib().mpkx_mlp_nparam())
+        params = np.ascontiguousarray(params, np.float32)
+        if params.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters, got {params.size}")
+        _check(lib().mpkx_mlp_create(C.byref(self._h), _p(params)))
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        if params.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters, got {params.size}")
+        _check(lib().mpkx_mlp_set_params(self._h, _p(params)))
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        _check(lib().mpkx_mlp_get(self._h, _p(out), None))
+        return out
+
+    def grad(self):
+        out = np.zeros(self.nparam, np.float32)
+        _check(lib().mpkx_mlp_get(self._h, None, _p(out)))
+        return out
+
+    def forward(self, X, Y, stream=None):
+        """X device FP32 [N][24] -> Y device FP32 [N][3]."""
+        _check(lib().mpkx_mlp_forward(self._h, int(X.shape[0]), _dev(X), _dev(Y), _stream(stream)))
+
+    def loss_grad(self, X, T, stream=None):
+        loss = C.c_float()
+        _check(lib().mpkx_mlp_loss_grad(self._h, int(X.shape[0]), _dev(X), _dev(T), C.byref(loss),
+                                        _stream(stream)))
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0, stream=None):
+        _check(lib().mpkx_mlp_adam(self._h, lr, beta1, beta2, eps, weight_decay, _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mpkx_mlp_destroy(self._h)
+            self._h = None

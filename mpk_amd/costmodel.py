@@ -1,0 +1,323 @@
+"""Autotuning and the learned cost model for the batched block-RHS MPK.
+
+The tuning configuration of mpkx_batch is (B, L, T): B right-hand sides per
+pass over the matrix (vector block width), L lanes splitting each row's
+entries, and the long-row threshold T above which a row gets a workgroup.
+
+* `space(nrhs)` lists every valid configuration; `tune` times all of them on
+  the GPU (the exhaustive autotuner, which is also the judge of any pick).
+* `features` (in the package) describes a matrix by 24 numbers.
+* `CostModel` is a 24 -> 64 -> 64 -> 3 MLP whose three heads predict
+  log2 B, log2 L and log2 T of the fastest configuration.  It is trained on
+  the GPU by the library's own forward/backward kernels (`mpk_amd.Mlp`) on
+  the autotuner's timings: the target of a head is the timing-weighted mean
+  of log2 of that parameter over all configurations, weight
+  exp(-(t/t_min - 1)/tau), so near-ties share the target.  A prediction is
+  mapped to the nearest valid configuration; when a matrix's features fall
+  outside the training range the exhaustive autotuner runs instead.
+"""
+import json
+import math
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+import mpk_amd as M
+
+WIDTHS = (1, 2, 4, 8, 16)
+LANES = (1, 2, 4, 8, 16, 32, 64)
+LONG_ROWS = (32, 128, 512, 2048)
+DEFAULT = (8, 1, 256)  # mpkx_batch_create's configuration (B = min(nrhs, 8))
+TAU = 0.03
+
+
+def default_config(nrhs):
+    return (min(nrhs, DEFAULT[0]), DEFAULT[1], DEFAULT[2])
+
+
+def space(nrhs):
+    """Every valid (B, L, T) for nrhs right-hand sides."""
+    return [(b, l, t) for b in WIDTHS if b <= nrhs and nrhs % b == 0
+            for l in LANES if b * l <= 64 for t in LONG_ROWS]
+
+
+def time_exec(bm, X, VV, reps=3, warmup=1):
+    import torch
+    for _ in range(warmup):
+        bm.exec(X, VV)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        bm.exec(X, VV)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(bm, X, VV, configs=None, reps=3):
+    """Times every configuration; returns {config: ms}."""
+    out = {}
+    for cfg in (configs or space(bm.nrhs)):
+        bm.configure(*cfg)
+        out[cfg] = time_exec(bm, X, VV, reps)
+    return out
+
+
+def soft_target(times):
+    """Timing-weighted mean of (log2 B, log2 L, log2 T) over the configurations."""
+    cfgs = list(times)
+    t = np.array([times[c] for c in cfgs])
+    w = np.exp(-(t / t.min() - 1.0) / TAU)
+    lg = np.log2(np.array(cfgs, dtype=np.float64))
+    return (w[:, None] * lg).sum(0) / w.sum()
+
+
+def nearest(pred, nrhs):
+    """Valid configuration closest (in log2 space) to a prediction."""
+    best, bd = None, math.inf
+    for cfg in space(nrhs):
+        d = float(np.sum((np.log2(cfg) - pred) ** 2))
+        if d < bd:
+            best, bd = cfg, d
+    return best
+
+
+# ---- synthetic training matrices ---------------------------------------------
+
+def suite(count, seed=1):
+    """Specs of `count` synthetic matrices: 2D/3D stencils, banded (bandwidth
+    10..10^4), Chung-Lu power law; rows log-uniform in [8k, 2M]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        kind = ("stencil", "banded", "chunglu")[i % 3]
+        nrhs = int(rng.choice([4, 8, 16], p=[0.2, 0.3, 0.5]))
+        n = int(math.exp(rng.uniform(math.log(8192), math.log(2097152))))
+        s = {"kind": kind, "nrhs": nrhs, "seed": int(rng.integers(1, 2**62))}
+        if kind == "stencil":
+            typ = str(rng.choice(["m5p", "m9p", "m7p", "cube"]))
+            side = max(8, int(round(n ** (1 / 3 if typ == "cube" else 1 / 2))))
+            s.update(type=typ, size=side)
+        elif kind == "banded":
+            s.update(n=n, bandwidth=int(math.exp(rng.uniform(math.log(10), math.log(1e4)))),
+                     per_row=int(rng.integers(3, 34)))
+        else:
+            s.update(n=n, exponent=float(rng.uniform(2.1, 3.0)), mean=float(rng.uniform(4.0, 30.0)))
+        out.append(s)
+    return out
+
+
+def make_matrix(spec):
+    if spec["kind"] == "stencil":
+        return M.gen2(spec["type"], spec["size"], -1.0)
+    if spec["kind"] == "banded":
+        return M.gen_banded(spec["n"], spec["bandwidth"], spec["per_row"], 1.0, spec["seed"])
+    return M.gen_chunglu(spec["n"], spec["mean"], spec["exponent"], 1.0, spec["seed"])
+
+
+def generate(specs, pool, window):
+    """Matrices of specs in order, at most `window` generated ahead."""
+    pending = []
+    for s in specs:
+        pending.append(pool.submit(make_matrix, s))
+        if len(pending) > window:
+            yield pending.pop(0).result()
+    for f in pending:
+        yield f.result()
+
+
+def collect(specs, nlevel=2, reps=3, threads=8, log=None):
+    """Features and exhaustive timings of every spec (GPU).  Returns
+    (features [N,24], list of {config: ms})."""
+    import torch
+    feats, timings = [], []
+    t0 = time.time()
+    with ThreadPoolExecutor(threads) as pool:
+        for i, (spec, (rp, ci, va)) in enumerate(zip(specs, generate(specs, pool, 2 * threads))):
+            nrhs = spec["nrhs"]
+            feats.append(M.features(rp, ci, nrhs))
+            bm = M.BatchMPK([(rp, ci, va)], nrhs, nlevel)
+            n = len(rp) - 1
+            X = torch.rand(n * nrhs, device="cuda:0", dtype=torch.float32) * 2 - 1
+            VV = torch.empty(n * nrhs * (nlevel + 1), device="cuda:0", dtype=torch.float32)
+            timings.append(tune(bm, X, VV, reps=reps))
+            del bm
+            if log and (i + 1) % 25 == 0:
+                log(f"collect {i + 1}/{len(specs)} {time.time() - t0:.0f}s")
+    return np.array(feats), timings
+
+
+# ---- the cost model ----------------------------------------------------------
+
+def init_params(seed=0):
+    """He-uniform initialisation of the 24-64-64-3 parameter vector."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((24, 64), (64, 64), (64, 3)):
+        lim = math.sqrt(6.0 / fan_in)
+        parts += [rng.uniform(-lim, lim, fan_out * fan_in), np.zeros(fan_out)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+class CostModel:
+    """Trained MLP + feature normalisation; predicts (B, L, T)."""
+
+    def __init__(self, params, mean, std, lo, hi, meta=None):
+        self.params = np.asarray(params, np.float32)
+        self.mean, self.std = np.asarray(mean), np.asarray(std)
+        self.lo, self.hi = np.asarray(lo), np.asarray(hi)  # training feature range
+        self.meta = meta or {}
+        self._mlp = None
+
+    def _net(self):
+        if self._mlp is None:
+            self._mlp = M.Mlp(self.params)
+        return self._mlp
+
+    def normalise(self, feats):
+        return ((np.atleast_2d(feats) - self.mean) / self.std).astype(np.float32)
+
+    def predict(self, feats):
+        """Head outputs (log2 B, log2 L, log2 T) for each feature row (GPU forward)."""
+        import torch
+        X = torch.from_numpy(self.normalise(feats)).to("cuda:0")
+        Y = torch.empty((X.shape[0], 3), dtype=torch.float32, device="cuda:0")
+        self._net().forward(X, Y)
+        return Y.cpu().numpy().astype(np.float64)
+
+    def in_range(self, feats, margin=0.25):
+        span = self.hi - self.lo
+        f = np.asarray(feats)
+        return bool(np.all(f >= self.lo - margin * span) and np.all(f <= self.hi + margin * span))
+
+    def pick(self, feats, nrhs):
+        return nearest(self.predict(feats)[0], nrhs)
+
+    def choose(self, ptr, col, nrhs, bm=None, X=None, VV=None):
+        """Configuration for a matrix: the network's pick, or the exhaustive
+        autotuner's when the features leave the training range (needs bm, X, VV)."""
+        f = M.features(ptr, col, nrhs)
+        if self.in_range(f) or bm is None:
+            return self.pick(f, nrhs), "model"
+        times = tune(bm, X, VV)
+        return min(times, key=times.get), "exhaustive"
+
+    def save(self, path):
+        with open(path, "w") as fh:
+            json.dump({"params": self.params.tolist(), "mean": self.mean.tolist(), "std": self.std.tolist(),
+                       "lo": self.lo.tolist(), "hi": self.hi.tolist(), "meta": self.meta}, fh)
+
+    @staticmethod
+    def load(path):
+        with open(path) as fh:
+            d = json.load(fh)
+        return CostModel(d["params"], d["mean"], d["std"], d["lo"], d["hi"], d.get("meta"))
+
+
+def train(feats, timings, steps=4000, lr=2e-3, weight_decay=1e-4, seed=0, log=None):
+    """Full-batch Adam on the GPU with the library's MLP kernels."""
+    import torch
+    feats = np.asarray(feats, np.float64)
+    targets = np.array([soft_target(t) for t in timings])
+    mean = feats.mean(0)
+    std = feats.std(0)
+    std[std < 1e-9] = 1.0
+    net = M.Mlp(init_params(seed))
+    X = torch.from_numpy(((feats - mean) / std).astype(np.float32)).to("cuda:0")
+    T = torch.from_numpy(targets.astype(np.float32)).to("cuda:0")
+    hist = []
+    for it in range(steps):
+        loss = net.loss_grad(X, T)
+        net.adam(lr * (0.1 ** (it / steps)), weight_decay=weight_decay)
+        if it % 500 == 0 or it == steps - 1:
+            hist.append((it, loss))
+            if log:
+                log(f"train step {it} loss {loss:.5f}")
+    model = CostModel(net.params(), mean, std, feats.min(0), feats.max(0),
+                      {"samples": len(timings), "steps": steps, "lr": lr, "final_loss": hist[-1][1]})
+    return model, hist
+
+
+def regret(timings, picks):
+    """time(pick) / time(best) for each sample."""
+    return np.array([t[p] / min(t.values()) for t, p in zip(timings, picks)])
+
+
+# ---- data files and command line -----------------------------------------------
+
+def save_data(path, specs, feats, timings):
+    """Features and timings as arrays: times[i, j] = ms of ALL[j] (NaN if invalid)."""
+    allc = space(16)
+    times = np.full((len(timings), len(allc)), np.nan, np.float32)
+    for i, t in enumerate(timings):
+        for j, c in enumerate(allc):
+            if c in t:
+                times[i, j] = t[c]
+    np.savez_compressed(path, feats=np.asarray(feats), times=times, configs=np.array(allc),
+                        specs=np.array(json.dumps(specs)))
+
+
+def load_data(path):
+    d = np.load(path)
+    allc = [tuple(int(v) for v in c) for c in d["configs"]]
+    timings = [{c: float(t) for c, t in zip(allc, row) if not np.isnan(t)} for row in d["times"]]
+    return json.loads(str(d["specs"])), d["feats"], timings
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m mpk_amd.costmodel")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect", help="exhaustive timings of a synthetic suite (GPU)")
+    c.add_argument("--count", type=int, default=2000)
+    c.add_argument("--seed", type=int, default=1)
+    c.add_argument("--start", type=int, default=0)
+    c.add_argument("--out", required=True)
+    t = sub.add_parser("train", help="train the MLP on collected timings (GPU)")
+    t.add_argument("--data", nargs="+", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--steps", type=int, default=4000)
+    t.add_argument("--holdout", type=float, default=0.1)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if a.cmd == "collect":
+        specs = suite(a.count, a.seed)[a.start:]
+        feats, timings = collect(specs, log=log)
+        save_data(a.out, specs, feats, timings)
+        log(f"wrote {a.out}: {len(specs)} matrices")
+        return
+    specs, feats, timings = [], [], []
+    for p in a.data:
+        s, f, t = load_data(p)
+        specs += s
+        feats.append(f)
+        timings += t
+    feats = np.concatenate(feats)
+    idx = np.random.default_rng(0).permutation(len(timings))
+    nh = int(len(idx) * a.holdout)
+    hold, tr = idx[:nh], idx[nh:]
+    model, hist = train(feats[tr], [timings[i] for i in tr], steps=a.steps, log=log)
+    out = {}
+    for name, sel in (("train", tr), ("holdout", hold)):
+        if len(sel) == 0:
+            continue
+        tm = [timings[i] for i in sel]
+        picks = [nearest(p, specs[i]["nrhs"]) for p, i in zip(model.predict(feats[sel]), sel)]
+        defaults = [default_config(specs[i]["nrhs"]) for i in sel]
+        rm, rd = regret(tm, picks), regret(tm, defaults)
+        out[name] = {"n": int(len(sel)), "model_mean_regret": float(rm.mean()),
+                     "model_max_regret": float(rm.max()), "default_mean_regret": float(rd.mean()),
+                     "default_max_regret": float(rd.max()),
+                     "model_within_5pct": float(np.mean(rm <= 1.05))}
+        log(f"{name}: {out[name]}")
+    model.meta.update({"evaluation": out, "loss_history": hist, "data": [str(p) for p in a.data]})
+    model.save(a.out)
+    log(f"wrote {a.out}")
+
+
+if __name__ == "__main__":
+    main()

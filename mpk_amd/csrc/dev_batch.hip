@@ -1,0 +1,307 @@
+// dev_batch.hip -- batched matrix powers with block right-hand sides.
+//
+// Many independent sparse matrices (uneven sizes, power-law rows), each with
+// R right-hand sides, advanced k powers by one hipGraph.  The matrices are
+// concatenated into one block-diagonal CSR with FP32 values pre-scaled by
+// 1/sigma_b (sigma_b = the power of two at or above matrix b's largest row
+// abs-sum: the scaling is exact and keeps every power in range).  Vectors
+// are row-major [row][rhs] FP32:  VV[l][row][r] = ((A_b/sigma_b)^l X_b)[row][r].
+//
+// Power-law hubs make skirt tiling useless (the distance-1 ball of a hub is
+// a large part of its graph), so every power is one block SpMM:
+//   group rows: B*L lanes per row; lane (l, r) owns right-hand side c0 + r
+//               and entries rp[row] + l, + L, ...; the L partial sums meet in
+//               a butterfly of lane shuffles.  The B lanes of one l read the
+//               same (col, val) and one contiguous B*4-byte piece of x[col].
+//   long rows:  (nnz > T) a 256-lane workgroup per row: 256/B groups stride
+//               the row and reduce in LDS.
+//   passes:     R/B passes over the matrix per power when B < R.
+// (B, L, T) is the tuning configuration.  A row's operation order depends
+// only on its own entries and (B, L, T), never on the other matrices, so a
+// matrix's result is bitwise the same batched or alone.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "dev_common.h"
+#include "internal.h"
+
+using namespace mpkx;
+
+namespace {
+
+constexpr int BT = 256;
+
+template <int B, int L>
+__global__ __launch_bounds__(BT) void k_spmm_group(const int *__restrict__ rows, int nrows,
+                                                   const int *__restrict__ rp, const int *__restrict__ ci,
+                                                   const float *__restrict__ va, const float *__restrict__ x,
+                                                   float *__restrict__ y, int ldv, int c0) {
+  constexpr int G = B * L;
+  static_assert(G <= 64 && (G & (G - 1)) == 0, "row group must fit a wavefront");
+  const long g = (long)blockIdx.x * BT + threadIdx.x;
+  const long slot = g / G;
+  const int t = (int)(g % G), r = t % B, l = t / B;
+  const bool active = slot < nrows;
+  const int row = active ? rows[slot] : 0;
+  float acc = 0.0f;
+  if (active) {
+    const int e = rp[row + 1];
+    const float *xc = x + c0 + r;
+    for (int j = rp[row] + l; j < e; j += L) acc = fmaf(va[j], xc[(size_t)ci[j] * ldv], acc);
+  }
+#pragma unroll
+  for (int off = G / 2; off >= B; off >>= 1) acc += __shfl_xor(acc, off, G);
+  if (active && l == 0) y[(size_t)row * ldv + c0 + r] = acc;
+}
+
+template <int B>
+__global__ __launch_bounds__(BT) void k_spmm_long(const int *__restrict__ rows, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci, const float *__restrict__ va,
+                                                  const float *__restrict__ x, float *__restrict__ y, int ldv,
+                                                  int c0) {
+  constexpr int G = BT / B;
+  __shared__ float red[BT];
+  const int row = rows[blockIdx.x];
+  const int grp = threadIdx.x / B, r = threadIdx.x % B;
+  const float *xc = x + c0 + r;
+  float acc = 0.0f;
+  for (int j = rp[row] + grp; j < rp[row + 1]; j += G) acc = fmaf(va[j], xc[(size_t)ci[j] * ldv], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = G / 2; w > 0; w >>= 1) {
+    if (grp < w) red[threadIdx.x] += red[threadIdx.x + w * B];
+    __syncthreads();
+  }
+  if (grp == 0) y[(size_t)row * ldv + c0 + r] = red[r];
+}
+
+struct LevelArgs {
+  const int *srows, *lrows, *rp, *ci;
+  const float *va;
+  int nshort, nlong, ldv;
+};
+
+template <int B, int L>
+void launch_group(const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
+  long threads = (long)a.nshort * B * L;
+  hipLaunchKernelGGL((k_spmm_group<B, L>), dim3((unsigned)((threads + BT - 1) / BT)), dim3(BT), 0, st, a.srows,
+                     a.nshort, a.rp, a.ci, a.va, x, y, a.ldv, c0);
+}
+
+template <int B>
+bool launch_group_l(int L, const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
+  switch (L) {
+    case 1: launch_group<B, 1>(a, x, y, c0, st); return true;
+    case 2: launch_group<B, 2>(a, x, y, c0, st); return true;
+    case 4: launch_group<B, 4>(a, x, y, c0, st); return true;
+    case 8: if constexpr (B * 8 <= 64) { launch_group<B, 8>(a, x, y, c0, st); return true; } break;
+    case 16: if constexpr (B * 16 <= 64) { launch_group<B, 16>(a, x, y, c0, st); return true; } break;
+    case 32: if constexpr (B * 32 <= 64) { launch_group<B, 32>(a, x, y, c0, st); return true; } break;
+    case 64: if constexpr (B * 64 <= 64) { launch_group<B, 64>(a, x, y, c0, st); return true; } break;
+  }
+  return false;
+}
+
+template <int B>
+bool launch_level_b(int L, const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
+  if (a.nshort > 0 && !launch_group_l<B>(L, a, x, y, c0, st)) return false;
+  if (a.nlong > 0)
+    hipLaunchKernelGGL((k_spmm_long<B>), dim3(a.nlong), dim3(BT), 0, st, a.lrows, a.rp, a.ci, a.va, x, y, a.ldv, c0);
+  return true;
+}
+
+bool launch_level(int B, int L, const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
+  switch (B) {
+    case 1: return launch_level_b<1>(L, a, x, y, c0, st);
+    case 2: return launch_level_b<2>(L, a, x, y, c0, st);
+    case 4: return launch_level_b<4>(L, a, x, y, c0, st);
+    case 8: return launch_level_b<8>(L, a, x, y, c0, st);
+    case 16: return launch_level_b<16>(L, a, x, y, c0, st);
+  }
+  return false;
+}
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+}  // namespace
+
+struct mpkx_batch {
+  int nmat = 0, nrhs = 0, nlevel = 0;
+  int B = 0, L = 0, T = 0, nshort = 0, nlong = 0;
+  long ntot = 0, nnz = 0;
+  std::vector<int> rp_host;  // row pointers of the concatenated matrix
+  DevArray<int> rp, ci, srows, lrows;
+  DevArray<float> va;
+  GraphCache graph;
+};
+
+static bool batch_launches(mpkx_batch *b, const float *X, float *VV, hipStream_t st) {
+  const size_t stride = (size_t)b->ntot * b->nrhs;
+  if ((const float *)VV != X) (void)hipMemcpyAsync(VV, X, sizeof(float) * stride, hipMemcpyDeviceToDevice, st);
+  LevelArgs a{b->srows.p, b->lrows.p, b->rp.p, b->ci.p, b->va.p, b->nshort, b->nlong, b->nrhs};
+  for (int l = 0; l < b->nlevel; l++)
+    for (int c0 = 0; c0 < b->nrhs; c0 += b->B)
+      if (!launch_level(b->B, b->L, a, VV + l * stride, VV + (l + 1) * stride, c0, st)) return false;
+  return true;
+}
+
+extern "C" {
+
+int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row, int long_row) {
+  clear_error();
+  if (!b) return fail(MPKX_EINVAL, "batch_configure: null");
+  if (!pow2(block_width) || block_width > 16 || b->nrhs % block_width != 0)
+    return fail(MPKX_EUNSUPPORTED, "batch_configure: block width %d (power of two <= 16 dividing nrhs %d)",
+                block_width, b->nrhs);
+  if (!pow2(lanes_per_row) || block_width * lanes_per_row > 64)
+    return fail(MPKX_EUNSUPPORTED, "batch_configure: lanes per row %d (power of two, width*lanes <= 64)",
+                lanes_per_row);
+  if (long_row < 1) return fail(MPKX_EINVAL, "batch_configure: long-row threshold %d", long_row);
+  std::vector<int> srows, lrows;
+  try {
+    srows.reserve(b->ntot);
+    for (long i = 0; i < b->ntot; i++) {
+      if (b->rp_host[i + 1] - b->rp_host[i] > long_row)
+        lrows.push_back((int)i);
+      else
+        srows.push_back((int)i);
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(MPKX_ENOMEM, "batch_configure: host allocation");
+  }
+  b->nshort = (int)srows.size();
+  b->nlong = (int)lrows.size();
+  if (srows.empty()) srows.push_back(0);
+  if (lrows.empty()) lrows.push_back(0);
+  int rc;
+  if ((rc = b->srows.upload(srows)) || (rc = b->lrows.upload(lrows))) return rc;
+  b->B = block_width;
+  b->L = lanes_per_row;
+  b->T = long_row;
+  b->graph.reset();
+  return MPKX_OK;
+}
+
+int mpkx_batch_create(mpkx_batch **out, int nmat, const int *ns, const int *const *ptrs, const int *const *cols,
+                      const double *const *vals, int nrhs, int nlevel, double *sigmas) {
+  clear_error();
+  if (!out || nmat <= 0 || !ns || !ptrs || !cols || !vals || nlevel <= 0)
+    return fail(MPKX_EINVAL, "batch_create: bad arguments");
+  if (!pow2(nrhs) || nrhs > 64) return fail(MPKX_EUNSUPPORTED, "batch_create: nrhs %d (power of two <= 64)", nrhs);
+  auto *b = new (std::nothrow) mpkx_batch;
+  if (!b) return fail(MPKX_ENOMEM, "batch_create");
+  std::unique_ptr<mpkx_batch> guard(b);
+  b->nmat = nmat;
+  b->nrhs = nrhs;
+  b->nlevel = nlevel;
+  long ntot = 0, nnz = 0;
+  for (int m = 0; m < nmat; m++) {
+    if (ns[m] <= 0 || !ptrs[m] || !cols[m] || !vals[m] || ptrs[m][0] != 0)
+      return fail(MPKX_EINVAL, "batch_create: matrix %d is empty or malformed", m);
+    ntot += ns[m];
+    nnz += ptrs[m][ns[m]];
+  }
+  if (nnz >= INT32_MAX || ntot * (long)nrhs >= INT32_MAX)
+    return fail(MPKX_EUNSUPPORTED, "batch_create: batch too large for 32-bit indices");
+  b->ntot = ntot;
+  b->nnz = nnz;
+  std::vector<long> roff(nmat + 1, 0), eoff(nmat + 1, 0);
+  for (int m = 0; m < nmat; m++) {
+    roff[m + 1] = roff[m] + ns[m];
+    eoff[m + 1] = eoff[m] + ptrs[m][ns[m]];
+  }
+  std::vector<int> ci;
+  std::vector<float> va;
+  try {
+    b->rp_host.assign(ntot + 1, 0);
+    ci.resize(nnz);
+    va.resize(nnz);
+  } catch (const std::bad_alloc &) {
+    return fail(MPKX_ENOMEM, "batch_create: host allocation");
+  }
+  std::vector<int> bad(nmat, 0);
+  parallel_for(nmat, [&](long m) {
+    const int *p = ptrs[m], *c = cols[m];
+    const double *v = vals[m];
+    double mx = 0.0;
+    for (int i = 0; i < ns[m]; i++) {
+      if (p[i + 1] < p[i]) bad[m] = 1;
+      double s = 0.0;
+      for (int j = p[i]; j < p[i + 1]; j++) s += std::fabs(v[j]);
+      mx = std::max(mx, s);
+    }
+    int e2 = 0;
+    if (mx > 0) {
+      double f = std::frexp(mx, &e2);
+      if (f == 0.5) e2 -= 1;
+    }
+    const double sigma = std::ldexp(1.0, e2);
+    if (sigmas) sigmas[m] = sigma;
+    const long r0 = roff[m], e0 = eoff[m];
+    for (int i = 0; i < ns[m]; i++) b->rp_host[r0 + i + 1] = (int)(e0 + p[i + 1]);
+    for (long j = 0; j < p[ns[m]]; j++) {
+      if (c[j] < 0 || c[j] >= ns[m]) bad[m] = 2;
+      ci[e0 + j] = (int)(r0 + c[j]);
+      va[e0 + j] = (float)(v[j] / sigma);
+    }
+  });
+  for (int m = 0; m < nmat; m++)
+    if (bad[m]) return fail(MPKX_EINVAL, "batch_create: matrix %d has %s", m, bad[m] == 1 ? "decreasing ptr" : "a column out of range");
+  int rc;
+  if ((rc = b->rp.upload(b->rp_host)) || (rc = b->ci.upload(ci)) || (rc = b->va.upload(va))) return rc;
+  if ((rc = mpkx_batch_configure(b, nrhs < 8 ? nrhs : 8, 1, 256))) return rc;
+  *out = guard.release();
+  return MPKX_OK;
+}
+
+void mpkx_batch_destroy(mpkx_batch *b) { delete b; }
+
+int mpkx_batch_exec(mpkx_batch *b, const float *X, float *VV, void *stream, int use_graph) {
+  clear_error();
+  if (!b || !X || !VV) return fail(MPKX_EINVAL, "batch_exec: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    if (!batch_launches(b, X, VV, st)) return fail(MPKX_EUNSUPPORTED, "batch_exec: configuration");
+    MPKX_HIP(hipGetLastError());
+    return MPKX_OK;
+  }
+  if (!b->graph.matches(X, VV, st, 0)) {
+    b->graph.reset();
+    hipStream_t cap = st;
+    bool own = false;
+    if (cap == nullptr) {
+      MPKX_HIP(hipStreamCreate(&cap));
+      own = true;
+    }
+    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    bool ok = batch_launches(b, X, VV, cap);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(cap, &g);
+    if (own) (void)hipStreamDestroy(cap);
+    if (!ok || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(ok ? MPKX_EHIP : MPKX_EUNSUPPORTED, "batch_exec: graph capture failed");
+    }
+    b->graph.graph = g;
+    MPKX_HIP(hipGraphInstantiate(&b->graph.exec, b->graph.graph, nullptr, nullptr, 0));
+    b->graph.k0 = X;
+    b->graph.k1 = VV;
+    b->graph.stream = st;
+    b->graph.k = 0;
+  }
+  MPKX_HIP(hipGraphLaunch(b->graph.exec, st));
+  return MPKX_OK;
+}
+
+int mpkx_batch_stats(const mpkx_batch *b, long *rows, long *nnz, long *long_rows, int *launches) {
+  clear_error();
+  if (!b) return fail(MPKX_EINVAL, "batch_stats: null");
+  if (rows) *rows = b->ntot;
+  if (nnz) *nnz = b->nnz;
+  if (long_rows) *long_rows = b->nlong;
+  if (launches) *launches = b->nlevel * (b->nrhs / b->B) * ((b->nshort > 0) + (b->nlong > 0));
+  return MPKX_OK;
+}
+
+}  // extern "C"

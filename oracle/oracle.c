@@ -38,6 +38,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <limits.h>
+#include <math.h>
 
 #define ORC_API __attribute__((visibility("default")))
 
@@ -1049,4 +1050,148 @@ ORC_API int orc_pd_exec(const char *prefix, int np, const double *x, double *y) 
   }
   free(b);
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batched block-RHS MPK in FP32 (no reference counterpart).  One      */
+/* matrix at a time; values scaled by 1/sigma, sigma = the power of    */
+/* two at or above the largest row abs-sum, then rounded to FP32.      */
+/* Row order of operations: rows with nnz <= T are split over L lanes  */
+/* (lane l sums entries l, l+L, ... with fmaf from 0), the L partial   */
+/* sums combined by a butterfly (for m = L/2 .. 1: a_l += a_{l xor m});*/
+/* rows with nnz > T over G = 256/B groups the same way, the group     */
+/* sums combined by a tree (for w = G/2 .. 1: a_g += a_{g+w}, g < w).  */
+/* X, VV row-major [row][nrhs]; VV level-major with X as level 0.      */
+/* ------------------------------------------------------------------ */
+ORC_API double orc_pow2_sigma(int n, const int *ptr, const double *val) {
+  double mx = 0.0;
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) s += val[j] < 0 ? -val[j] : val[j];
+    if (s > mx) mx = s;
+  }
+  double sigma = 1.0;
+  if (mx == 0.0) return sigma;
+  while (sigma < mx) sigma *= 2.0;
+  while (sigma / 2.0 >= mx) sigma /= 2.0;
+  return sigma;
+}
+
+/* Pairwise tree of the lane shuffles and of the LDS reduction alike:  */
+/* for w = cnt/2 .. 1: a[g] = a[g] + a[g + w] for g < w (cnt a power   */
+/* of two; the xor butterfly gives lane 0 the same sums).              */
+static float orc_tree32(float *a, int cnt) {
+  for (int w = cnt / 2; w > 0; w >>= 1)
+    for (int g = 0; g < w; g++) a[g] = a[g] + a[g + w];
+  return a[0];
+}
+
+static float orc_row32(const int *col, const float *v, int b, int e, const float *x, int ldv,
+                       int c, int lanes) {
+  float part[256];
+  for (int l = 0; l < lanes; l++) {
+    float a = 0.0f;
+    for (int j = b + l; j < e; j += lanes) a = fmaf(v[j], x[(long)col[j] * ldv + c], a);
+    part[l] = a;
+  }
+  return orc_tree32(part, lanes);
+}
+
+ORC_API int orc_batch_mpk32(int n, const int *ptr, const int *col, const double *val, int nrhs,
+                            int nlevel, int block_width, int lanes, int long_row,
+                            const float *X, float *VV, double *sigma_out) {
+  if (lanes < 1 || lanes > 64 || block_width < 1 || block_width > 256) return -1;
+  double sigma = orc_pow2_sigma(n, ptr, val);
+  if (sigma_out) *sigma_out = sigma;
+  long nnz = ptr[n];
+  float *v = malloc(sizeof(float) * (nnz ? nnz : 1));
+  for (long j = 0; j < nnz; j++) v[j] = (float)(val[j] / sigma);
+  long stride = (long)n * nrhs;
+  memcpy(VV, X, sizeof(float) * stride);
+  for (int l = 0; l < nlevel; l++) {
+    const float *x = VV + l * stride;
+    float *y = VV + (l + 1) * stride;
+    for (int i = 0; i < n; i++) {
+      int big = ptr[i + 1] - ptr[i] > long_row;
+      for (int c = 0; c < nrhs; c++)
+        y[(long)i * nrhs + c] = orc_row32(col, v, ptr[i], ptr[i + 1], x, nrhs, c,
+                                          big ? 256 / block_width : lanes);
+    }
+  }
+  free(v);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Cost-model MLP in FP64 (no reference counterpart): 24 -> 64 -> 64   */
+/* -> 3, ReLU on both hidden layers, linear heads.  Parameter vector:  */
+/*   W1[64][24], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]          */
+/* (row-major, out x in).  loss = 1/(2N) sum_n sum_o (y_no - t_no)^2.  */
+/* Returns the loss; y (N x 3) and grad (same layout as the parameters)*/
+/* are written when non-NULL.                                          */
+/* ------------------------------------------------------------------ */
+#define ORC_NI 24
+#define ORC_NH 64
+#define ORC_NO 3
+
+ORC_API long orc_mlp_nparam(void) {
+  return (long)ORC_NH * ORC_NI + ORC_NH + ORC_NH * ORC_NH + ORC_NH + ORC_NO * ORC_NH + ORC_NO;
+}
+
+ORC_API double orc_mlp_loss_grad(int N, const double *X, const double *T, const double *prm,
+                                 double *y_out, double *grad) {
+  const double *W1 = prm, *b1 = W1 + ORC_NH * ORC_NI, *W2 = b1 + ORC_NH, *b2 = W2 + ORC_NH * ORC_NH,
+               *W3 = b2 + ORC_NH, *b3 = W3 + ORC_NO * ORC_NH;
+  double *gW1 = grad, *gb1 = NULL, *gW2 = NULL, *gb2 = NULL, *gW3 = NULL, *gb3 = NULL;
+  if (grad) {
+    memset(grad, 0, sizeof(double) * orc_mlp_nparam());
+    gb1 = gW1 + ORC_NH * ORC_NI; gW2 = gb1 + ORC_NH; gb2 = gW2 + ORC_NH * ORC_NH;
+    gW3 = gb2 + ORC_NH; gb3 = gW3 + ORC_NO * ORC_NH;
+  }
+  double loss = 0.0;
+  for (int s = 0; s < N; s++) {
+    const double *x = X + (long)s * ORC_NI;
+    double h1[ORC_NH], h2[ORC_NH], y[ORC_NO], dy[ORC_NO], d2[ORC_NH], d1[ORC_NH];
+    for (int j = 0; j < ORC_NH; j++) {
+      double a = b1[j];
+      for (int i = 0; i < ORC_NI; i++) a += W1[j * ORC_NI + i] * x[i];
+      h1[j] = a > 0.0 ? a : 0.0;
+    }
+    for (int j = 0; j < ORC_NH; j++) {
+      double a = b2[j];
+      for (int i = 0; i < ORC_NH; i++) a += W2[j * ORC_NH + i] * h1[i];
+      h2[j] = a > 0.0 ? a : 0.0;
+    }
+    for (int o = 0; o < ORC_NO; o++) {
+      double a = b3[o];
+      for (int j = 0; j < ORC_NH; j++) a += W3[o * ORC_NH + j] * h2[j];
+      y[o] = a;
+      if (y_out) y_out[(long)s * ORC_NO + o] = a;
+      double r = a - T[(long)s * ORC_NO + o];
+      loss += 0.5 * r * r / N;
+      dy[o] = r / N;
+    }
+    if (!grad) continue;
+    for (int o = 0; o < ORC_NO; o++) {
+      gb3[o] += dy[o];
+      for (int j = 0; j < ORC_NH; j++) gW3[o * ORC_NH + j] += dy[o] * h2[j];
+    }
+    for (int j = 0; j < ORC_NH; j++) {
+      double a = 0.0;
+      for (int o = 0; o < ORC_NO; o++) a += W3[o * ORC_NH + j] * dy[o];
+      d2[j] = h2[j] > 0.0 ? a : 0.0;
+      gb2[j] += d2[j];
+      for (int i = 0; i < ORC_NH; i++) gW2[j * ORC_NH + i] += d2[j] * h1[i];
+    }
+    for (int i = 0; i < ORC_NH; i++) {
+      double a = 0.0;
+      for (int j = 0; j < ORC_NH; j++) a += W2[j * ORC_NH + i] * d2[j];
+      d1[i] = h1[i] > 0.0 ? a : 0.0;
+    }
+    for (int j = 0; j < ORC_NH; j++) {
+      gb1[j] += d1[j];
+      for (int i = 0; i < ORC_NI; i++) gW1[j * ORC_NI + i] += d1[j] * x[i];
+    }
+  }
+  return loss;
 }

@@ -43,6 +43,13 @@ def lib():
         L.orc_vexec_tasks.argtypes = [C.c_int, P, P, P, C.c_long, P, P]
         L.orc_pd_exec.argtypes = [C.c_char_p, C.c_int, P, P]
         L.orc_pd_exec.restype = C.c_int
+        L.orc_pow2_sigma.argtypes = [C.c_int, P, P]
+        L.orc_pow2_sigma.restype = C.c_double
+        L.orc_batch_mpk32.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
+        L.orc_batch_mpk32.restype = C.c_int
+        L.orc_mlp_nparam.restype = C.c_long
+        L.orc_mlp_loss_grad.argtypes = [C.c_int, P, P, P, P, P]
+        L.orc_mlp_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 
@@ -218,3 +225,39 @@ def case_inputs(case):
     parts = [read_ints(os.path.join(base, f"g{ph}.part.{npart}")) for ph in range(max(nphase, 1))]
     levs = [read_ints(os.path.join(base, f"l{ph}")) for ph in range(nphase)]
     return d, rp, ci, va, parts, levs
+
+
+def pow2_sigma(rp, va):
+    return lib().orc_pow2_sigma(len(rp) - 1, ptr(np.ascontiguousarray(rp, np.int32)),
+                                ptr(np.ascontiguousarray(va, np.float64)))
+
+
+def batch_mpk32(rp, ci, va, X, nrhs, nlevel, block_width, lanes, long_row):
+    """FP32 batched block-RHS MPK of one matrix; X float32 [n*nrhs]; returns
+    (VV float32 [(nlevel+1)*n*nrhs], sigma)."""
+    n = len(rp) - 1
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    va = np.ascontiguousarray(va, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    VV = np.zeros((nlevel + 1) * n * nrhs, np.float32)
+    sig = C.c_double()
+    rc = lib().orc_batch_mpk32(n, ptr(rp), ptr(ci), ptr(va), nrhs, nlevel, block_width, lanes, long_row,
+                               ptr(X), ptr(VV), C.byref(sig))
+    assert rc == 0
+    return VV, sig.value
+
+
+def mlp_nparam():
+    return int(lib().orc_mlp_nparam())
+
+
+def mlp_loss_grad(X, T, params):
+    """FP64 loss, outputs [N,3] and gradient of the 24-64-64-3 MLP."""
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    params = np.ascontiguousarray(params, np.float64)
+    y = np.zeros((X.shape[0], 3))
+    g = np.zeros(mlp_nparam())
+    loss = lib().orc_mlp_loss_grad(X.shape[0], ptr(X), ptr(T), ptr(params), ptr(y), ptr(g))
+    return loss, y, g

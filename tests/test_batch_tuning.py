@@ -1,0 +1,233 @@
+"""CPU tests of the batched block-RHS form and the learned tuning pieces:
+synthetic generators, matrix features (closed forms), the FP32 batched
+restatement in the oracle (exact on integer data, stated error otherwise,
+documented summation order) and the FP64 MLP oracle (finite differences)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mpk_amd as M
+from mpk_amd import costmodel as CM
+from tests import oracle_api as O
+
+
+def test_chunglu_properties():
+    rp, ci, va = M.gen_chunglu(5000, 10.0, 2.5, 1.0, seed=11)
+    n = len(rp) - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    P = (A != 0).astype(np.int8)
+    assert abs(P - P.T).sum() == 0, "pattern must be symmetric"
+    for i in range(n):
+        c = ci[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(c) > 0), f"row {i} columns not strictly increasing"
+        assert i in c
+    d = A.diagonal()
+    off = np.asarray(abs(A).sum(1)).ravel() - np.abs(d)
+    np.testing.assert_allclose(d - off, 1.0, rtol=0, atol=1e-9)
+    deg = np.diff(rp)
+    assert 8.0 < deg.mean() < 10.5
+    assert deg.max() > 10 * deg.mean(), "power law: hubs expected"
+    rp2, ci2, va2 = M.gen_chunglu(5000, 10.0, 2.5, 1.0, seed=11)
+    assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and np.array_equal(va, va2)
+    rp3, _, _ = M.gen_chunglu(5000, 10.0, 2.5, 1.0, seed=12)
+    assert not np.array_equal(rp, rp3)
+
+
+def test_banded_properties():
+    rp, ci, va = M.gen_banded(3000, 100, 7, 0.5, seed=3)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert np.abs(ci - rows).max() <= 100
+    assert np.diff(rp)[500] == 7  # interior row: all 7 offsets (0 among them)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    d = A.diagonal()
+    off = np.asarray(abs(A).sum(1)).ravel() - np.abs(d)
+    np.testing.assert_allclose(d - off, 0.5, atol=1e-9)
+
+
+def test_generator_errors():
+    with pytest.raises(M.MpkxError):
+        M.gen_chunglu(100, 2.0, 2.5)  # fewer than the 3 diagonal+ring entries
+    with pytest.raises(M.MpkxError):
+        M.gen_chunglu(100, 10.0, 1.5)
+    with pytest.raises(M.MpkxError):
+        M.gen_banded(100, 0, 3)
+
+
+def test_features_tridiagonal_closed_form():
+    n = 4096
+    rp, ci, _ = M.gen_banded(n, 1, 3, 1.0)  # offsets -1, 0, +1
+    f = M.features(rp, ci, 8)
+    nnz = 3 * n - 2
+    assert f[0] == pytest.approx(np.log2(n))
+    assert f[1] == pytest.approx(np.log2(nnz))
+    assert f[2] == pytest.approx(nnz / n)
+    assert f[4] == pytest.approx(np.log2(3)) and f[5] == 2
+    assert f[9] == 0 and f[10] == 0 and f[11] == 0
+    assert f[13] == pytest.approx(np.log2(1 + 2 * (n - 1) / nnz))
+    assert f[14] == pytest.approx(1.0) and f[15] == 1.0 and f[16] == 1.0
+    # blocks of b = n/4 rows at 0, (n-b)/3, 2(n-b)/3, n-b: the two end blocks
+    # grow by d rows, the two inner ones by 2d
+    b = n // 4
+    for d in range(1, 5):
+        assert f[16 + d] == pytest.approx(np.log2(1 + 1.5 * d / b))
+    assert f[21] == 3 and f[22] == pytest.approx(np.log2(4 * n * 8))
+    assert f[23] == pytest.approx(np.log2(8 * nnz + 4 * n))
+
+
+def test_features_cube():
+    rp, ci, _ = M.gen2("cube", 16, -1.0)
+    f = M.features(rp, ci, 1)
+    assert f[0] == pytest.approx(12.0)
+    assert f[4] == pytest.approx(np.log2(7)) and f[5] == 4 and f[6] == 7
+    assert f[21] == 0
+    # skirt growth of a block of 1024 rows = 4 full planes of the 16^3 cube:
+    # each level adds one plane on each open side
+    assert f[17] > 0 and np.all(np.diff(f[17:21]) > 0)
+
+
+def _scaled64(rp, ci, va):
+    sigma = O.pow2_sigma(rp, va)
+    return va / sigma, sigma
+
+
+def test_pow2_sigma():
+    rp = np.array([0, 2, 3], np.int32)
+    va = np.array([3.0, -1.0, 2.0])
+    assert O.pow2_sigma(rp, va) == 4.0
+    va = np.array([3.0, -1.0, 4.0])
+    assert O.pow2_sigma(rp, va) == 4.0  # exactly a power of two stays
+    va = np.array([0.1, 0.2, 0.05])
+    assert O.pow2_sigma(rp, va) == 0.5
+
+
+def _int_matrix(n, seed):
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        deg = int(rng.integers(1, 8)) if i % 17 else 60  # some long rows
+        cols = np.unique(np.append(rng.integers(0, n, deg), i))
+        rows.append(cols)
+    rp = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    va = rng.integers(-2, 3, len(ci)).astype(np.float64)
+    return rp, ci, va
+
+
+@pytest.mark.parametrize("cfg", [(8, 1, 256), (4, 4, 32), (1, 64, 8), (2, 16, 2048), (8, 8, 16)])
+def test_oracle_batch32_exact_on_integers(cfg):
+    """Integer matrix and vectors: every FP32 sum is exact, so any summation
+    order must reproduce the FP64 powers of A/sigma exactly; a wrong column,
+    right-hand side or scaling fails."""
+    rp, ci, va = _int_matrix(300, 5)
+    n, nrhs, k = len(rp) - 1, 8, 2
+    rng = np.random.default_rng(1)
+    X = rng.integers(-3, 4, (n, nrhs)).astype(np.float32)
+    VV, sigma = O.batch_mpk32(rp, ci, va, X.ravel(), nrhs, k, *cfg)
+    assert sigma == O.pow2_sigma(rp, va)
+    vs = va / sigma
+    for c in range(nrhs):
+        ref = O.spmv_k(rp, ci, vs, X[:, c].astype(np.float64), k)
+        got = VV.reshape(k + 1, n, nrhs)[:, :, c].ravel().astype(np.float64)
+        assert np.array_equal(got, ref), f"column {c}"
+
+
+def test_oracle_batch32_summation_order():
+    """One row [1, -1, e, e] (e = 1e-8) times ones, sigma = 4: sequential
+    fmaf gives e/4 + e/4 = 2e/4 only through the lanes' grouping."""
+    rp = np.array([0, 4, 5, 6, 7, 8], np.int32)
+    ci = np.array([1, 2, 3, 4, 1, 2, 3, 4], np.int32)
+    e = 1e-8
+    va = np.array([1.0, -1.0, e, e, 1, 1, 1, 1])
+    X = np.ones(5, np.float32)
+    f = np.float32
+    v = (va[:4] / 4).astype(f)
+
+    def run(L, T):
+        return O.batch_mpk32(rp, ci, va, X, 1, 1, 1, L, T)[0][5]
+
+    seq = ((f(0) + v[0]) + v[1]) + v[2] + v[3]
+    assert run(1, 64) == seq  # one lane: entries in order
+    two = (v[0] + v[2]) + (v[1] + v[3])
+    assert run(2, 64) == two  # lane 0: {1, e}, lane 1: {-1, e}
+    four = (v[0] + v[2]) + (v[1] + v[3])
+    assert run(4, 64) == four
+    assert seq != two  # the data separates the orders
+    # long row path (T=2 < 4 entries): 256 groups, tree over groups = same as L=4
+    assert run(1, 2) == four
+
+
+def test_oracle_batch32_error_vs_fp64():
+    rp, ci, va = M.gen_chunglu(3000, 10.0, 2.5, 1.0, seed=4)
+    n, nrhs, k = len(rp) - 1, 4, 6
+    X = np.random.default_rng(2).uniform(-1, 1, (n, nrhs)).astype(np.float32)
+    VV, sigma = O.batch_mpk32(rp, ci, va, X.ravel(), nrhs, k, 4, 1, 256)
+    vs = va / sigma
+    for c in range(nrhs):
+        ref = O.spmv_k(rp, ci, vs, X[:, c].astype(np.float64), k).reshape(k + 1, n)
+        got = VV.reshape(k + 1, n, nrhs)[:, :, c]
+        for l in range(1, k + 1):
+            err = np.linalg.norm(got[l] - ref[l]) / np.linalg.norm(ref[l])
+            assert err <= 1e-5, (c, l, err)
+
+
+def test_mlp_oracle_finite_differences():
+    rng = np.random.default_rng(0)
+    N = 6
+    X = rng.normal(size=(N, 24))
+    T = rng.normal(size=(N, 3))
+    p = CM.init_params(3).astype(np.float64)
+    p[-3:] = rng.normal(size=3)
+    p[24 * 64:24 * 64 + 64] = rng.normal(0, 0.1, 64)  # b1
+    loss, y, g = O.mlp_loss_grad(X, T, p)
+    assert loss == pytest.approx(0.5 * np.sum((y - T) ** 2) / N)
+    h = 1e-6
+    idx = list(rng.choice(len(p), 60, replace=False)) + [len(p) - 1, 24 * 64, 0]
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (O.mlp_loss_grad(X, T, pp)[0] - O.mlp_loss_grad(X, T, pm)[0]) / (2 * h)
+        assert fd == pytest.approx(g[i], rel=1e-5, abs=1e-8), i
+
+
+def test_mlp_oracle_closed_form():
+    """All weights zero: y = b3, the b3 gradient is mean(y - t)."""
+    p = np.zeros(O.mlp_nparam())
+    p[-3:] = [1.0, 2.0, 3.0]
+    X = np.ones((4, 24))
+    T = np.zeros((4, 3))
+    loss, y, g = O.mlp_loss_grad(X, T, p)
+    assert np.all(y == [1.0, 2.0, 3.0])
+    assert loss == pytest.approx(0.5 * 14)
+    assert np.allclose(g[-3:], [1.0, 2.0, 3.0]) and np.all(g[:-3] == 0)
+    assert O.mlp_nparam() == M.lib().mpkx_mlp_nparam() == 5955
+
+
+def test_config_space_and_targets():
+    assert len(CM.space(16)) == 25 * len(CM.LONG_ROWS)
+    assert all(b * l <= 64 and 16 % b == 0 for b, l, _ in CM.space(16))
+    assert all(b <= 4 for b, _, _ in CM.space(4))
+    for cfg in CM.space(8):
+        assert CM.nearest(np.log2(cfg), 8) == cfg
+    times = {c: 1.0 for c in CM.space(8)}
+    times[(4, 2, 128)] = 0.5
+    np.testing.assert_allclose(CM.soft_target(times), np.log2([4, 2, 128]), atol=1e-6)
+    assert CM.default_config(4) == (4, 1, 256)
+
+
+def test_cost_model_roundtrip(tmp_path):
+    m = CM.CostModel(CM.init_params(1), np.zeros(24), np.ones(24), -np.ones(24), np.ones(24), {"a": 1})
+    path = tmp_path / "cm.json"
+    m.save(path)
+    m2 = CM.CostModel.load(path)
+    assert np.array_equal(m.params, m2.params) and m2.meta == {"a": 1}
+    assert m2.in_range(np.zeros(24)) and not m2.in_range(np.full(24, 5.0))
+
+
+def test_suite_is_reproducible():
+    a, b = CM.suite(30, seed=4), CM.suite(30, seed=4)
+    assert a == b
+    assert {s["kind"] for s in a} == {"stencil", "banded", "chunglu"}
+    rp, ci, _ = CM.make_matrix(a[1])
+    assert len(rp) - 1 == a[1]["n"]

@@ -1,0 +1,226 @@
+"""GPU tests of the batched block-RHS MPK and the cost-model kernels.
+
+Batched FP32 MPK: bitwise equal to the oracle's FP32 restatement (same
+summation order, fmaf on both sides), bitwise equal between a batch and each
+matrix run alone, error vs the FP64 oracle stated per power (<= 1e-5).
+MLP: forward, loss and gradient of the HIP kernels against the FP64 oracle
+(whose gradient is itself checked by finite differences in the CPU suite)."""
+import os
+
+import numpy as np
+import pytest
+
+import mpk_amd as M
+from mpk_amd import costmodel as CM
+import tests.oracle_api as O
+from tests.gpu_util import say, to_host, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _mats(count, seed, lo=1500, hi=6000, mean=10.0):
+    rng = np.random.default_rng(seed)
+    return [M.gen_chunglu(int(rng.integers(lo, hi)), mean, 2.5, 1.0, seed=int(rng.integers(1, 2**40)))
+            for _ in range(count)]
+
+
+def _x(rows, nrhs, seed):
+    return np.random.default_rng(seed).uniform(-1, 1, rows * nrhs).astype(np.float32)
+
+
+def _run(bm, X, use_graph=True):
+    torch = torch_cuda()
+    Xd = torch.from_numpy(X).to("cuda")
+    VV = torch.zeros(X.size * (bm.nlevel + 1), dtype=torch.float32, device="cuda")
+    bm.exec(Xd, VV, use_graph=use_graph)
+    return to_host(VV)
+
+
+def _split(VV, bm, m):
+    """Level-major block of matrix m: [(k+1), n_m, nrhs]."""
+    r0, r1 = bm.row_offsets[m], bm.row_offsets[m + 1]
+    return VV.reshape(bm.nlevel + 1, bm.rows, bm.nrhs)[:, r0:r1, :]
+
+
+@pytest.mark.parametrize("cfg", [(8, 1, 256), (4, 4, 32), (2, 8, 128), (1, 64, 2048), (8, 8, 16), (2, 2, 32)])
+def test_batch_bitwise_vs_oracle(cfg):
+    say("batch vs oracle", cfg)
+    mats = _mats(3, 7)
+    nrhs, k = 8, 3
+    bm = M.BatchMPK(mats, nrhs, k)
+    bm.configure(*cfg)
+    X = _x(bm.rows, nrhs, 1)
+    VV = _run(bm, X)
+    Xr = X.reshape(bm.rows, nrhs)
+    for m, (rp, ci, va) in enumerate(mats):
+        r0, r1 = bm.row_offsets[m], bm.row_offsets[m + 1]
+        ref, sigma = O.batch_mpk32(rp, ci, va, Xr[r0:r1].ravel(), nrhs, k, *cfg)
+        assert bm.sigmas[m] == sigma
+        got = _split(VV, bm, m)
+        assert np.array_equal(got.ravel(), ref), f"matrix {m} config {cfg}"
+
+
+def test_batched_equals_unbatched():
+    say("batched == unbatched")
+    mats = _mats(5, 9, lo=800, hi=9000)
+    nrhs, k = 8, 6
+    bm = M.BatchMPK(mats, nrhs, k)
+    bm.configure(4, 2, 128)
+    X = _x(bm.rows, nrhs, 2)
+    VV = _run(bm, X)
+    Xr = X.reshape(bm.rows, nrhs)
+    for m in range(len(mats)):
+        r0, r1 = bm.row_offsets[m], bm.row_offsets[m + 1]
+        one = M.BatchMPK([mats[m]], nrhs, k)
+        one.configure(4, 2, 128)
+        alone = _run(one, np.ascontiguousarray(Xr[r0:r1]).ravel())
+        assert np.array_equal(_split(VV, bm, m).ravel(), alone), m
+
+
+def test_batch_columns_equal_single_vector():
+    """Short rows only (T above every row): column c of an 8-RHS run equals the
+    1-RHS run on column c."""
+    say("columns vs single vector")
+    mats = _mats(2, 3, mean=6.0)
+    k = 4
+    bm8 = M.BatchMPK(mats, 8, k)
+    bm8.configure(8, 2, 1 << 30)
+    X = _x(bm8.rows, 8, 3)
+    VV8 = _run(bm8, X).reshape(k + 1, bm8.rows, 8)
+    bm1 = M.BatchMPK(mats, 1, k)
+    bm1.configure(1, 2, 1 << 30)
+    for c in range(8):
+        VV1 = _run(bm1, np.ascontiguousarray(X.reshape(-1, 8)[:, c]))
+        assert np.array_equal(VV8[:, :, c].ravel(), VV1), c
+
+
+def test_batch_error_vs_fp64_per_power():
+    say("batch error per power")
+    mats = _mats(4, 21, lo=2000, hi=20000)
+    nrhs, k = 8, 6
+    bm = M.BatchMPK(mats, nrhs, k)
+    X = _x(bm.rows, nrhs, 4)
+    VV = _run(bm, X)
+    Xr = X.reshape(bm.rows, nrhs).astype(np.float64)
+    worst = np.zeros(k + 1)
+    for m, (rp, ci, va) in enumerate(mats):
+        r0, r1 = bm.row_offsets[m], bm.row_offsets[m + 1]
+        got = _split(VV, bm, m)
+        vs = va / bm.sigmas[m]
+        for c in range(nrhs):
+            ref = O.spmv_k(rp, ci, vs, Xr[r0:r1, c].copy(), k).reshape(k + 1, -1)
+            for l in range(k + 1):
+                e = np.linalg.norm(got[l, :, c] - ref[l]) / np.linalg.norm(ref[l])
+                worst[l] = max(worst[l], e)
+    say("per-power worst normwise error", worst)
+    assert worst[0] == 0 and np.all(worst <= 1e-5)
+
+
+def test_batch_graph_equals_eager_and_replays():
+    say("graph vs eager")
+    mats = _mats(3, 5)
+    bm = M.BatchMPK(mats, 16, 3)
+    bm.configure(4, 4, 64)
+    X = _x(bm.rows, 16, 5)
+    eager = _run(bm, X, use_graph=False)
+    torch = torch_cuda()
+    Xd = torch.from_numpy(X).to("cuda")
+    VV = torch.zeros(X.size * 4, dtype=torch.float32, device="cuda")
+    for rep in range(3):
+        VV.zero_()
+        bm.exec(Xd, VV)
+        assert np.array_equal(to_host(VV), eager), rep
+    # 16 right-hand sides in 4 passes of 4 against the oracle
+    rp, ci, va = mats[0]
+    ref, _ = O.batch_mpk32(rp, ci, va, X.reshape(bm.rows, 16)[:bm.row_offsets[1]].ravel(), 16, 3, 4, 4, 64)
+    assert np.array_equal(_split(eager, bm, 0).ravel(), ref)
+
+
+def test_batch_stats_and_errors():
+    mats = _mats(2, 1)
+    bm = M.BatchMPK(mats, 8, 2)
+    st = bm.stats()
+    assert st["rows"] == sum(len(m[0]) - 1 for m in mats)
+    assert st["nnz"] == sum(int(m[0][-1]) for m in mats)
+    with pytest.raises(M.MpkxError):
+        bm.configure(16, 1, 64)  # wider than nrhs
+    with pytest.raises(M.MpkxError):
+        bm.configure(8, 16, 64)  # 128 lanes per row
+    with pytest.raises(M.MpkxError):
+        M.BatchMPK(mats, 3, 2)
+
+
+# ---- cost-model kernels ------------------------------------------------------
+
+def _mlp_case(N, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(N, 24)).astype(np.float32)
+    T = rng.normal(size=(N, 3)).astype(np.float32)
+    p = CM.init_params(seed)
+    p[24 * 64:24 * 64 + 64] = rng.normal(0, 0.1, 64)
+    p[-3:] = rng.normal(size=3)
+    return X, T, p
+
+
+@pytest.mark.parametrize("N", [1, 37, 1000])
+def test_mlp_forward_loss_grad_vs_oracle(N):
+    say("mlp grad", N)
+    torch = torch_cuda()
+    X, T, p = _mlp_case(N, N)
+    net = M.Mlp(p)
+    Xd, Td = torch.from_numpy(X).to("cuda"), torch.from_numpy(T).to("cuda")
+    loss = net.loss_grad(Xd, Td)
+    g = net.grad()
+    Y = torch.empty((N, 3), dtype=torch.float32, device="cuda")
+    net.forward(Xd, Y)
+    y = to_host(Y)
+    loss64, y64, g64 = O.mlp_loss_grad(X.astype(np.float64), T.astype(np.float64), p.astype(np.float64))
+    assert np.linalg.norm(y - y64) / np.linalg.norm(y64) <= 1e-5
+    assert loss == pytest.approx(loss64, rel=1e-5)
+    assert np.linalg.norm(g - g64) / np.linalg.norm(g64) <= 1e-4
+    for lo, hi in ((0, 1536), (1536, 1600), (1600, 5696), (5696, 5760), (5760, 5952), (5952, 5955)):
+        assert np.linalg.norm(g[lo:hi] - g64[lo:hi]) <= 1e-4 * max(np.linalg.norm(g64[lo:hi]), 1e-6), (lo, hi)
+
+
+def test_mlp_adam_trains():
+    """A smooth target of the features: Adam on the GPU kernels cuts the loss
+    by 20x, and the result is deterministic."""
+    say("mlp train")
+    torch = torch_cuda()
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(512, 24)).astype(np.float32)
+    T = np.stack([np.tanh(X[:, 0] + X[:, 1]), X[:, 2] * 0.5, np.maximum(X[:, 3], 0)], 1).astype(np.float32)
+    Xd, Td = torch.from_numpy(X).to("cuda"), torch.from_numpy(T).to("cuda")
+    finals = []
+    for rep in range(2):
+        net = M.Mlp(CM.init_params(0))
+        first = net.loss_grad(Xd, Td)
+        for _ in range(400):
+            net.loss_grad(Xd, Td)
+            net.adam(3e-3)
+        finals.append((net.loss_grad(Xd, Td), net.params()))
+    assert finals[0][0] < first / 20
+    assert finals[0][0] == finals[1][0] and np.array_equal(finals[0][1], finals[1][1])
+
+
+def test_cost_model_gain_on_held_out():
+    """The committed model's picks on held-out matrices (a suite seed not used
+    for training) against the fixed default configuration, both judged by
+    the exhaustive autotuner."""
+    path = os.path.join(REPO, "mpk_amd", "cost_model.json")
+    if not os.path.exists(path):
+        pytest.skip("no trained cost model committed")
+    torch = torch_cuda()
+    model = CM.CostModel.load(path)
+    specs = CM.suite(24, seed=0xE7A1)
+    feats, timings = CM.collect(specs, nlevel=2, reps=3, threads=4)
+    picks = [CM.nearest(p, s["nrhs"]) for p, s in zip(model.predict(feats), specs)]
+    defaults = [CM.default_config(s["nrhs"]) for s in specs]
+    rm, rd = CM.regret(timings, picks), CM.regret(timings, defaults)
+    say(f"held-out regret: model mean {rm.mean():.3f} max {rm.max():.3f}; "
+        f"default mean {rd.mean():.3f} max {rd.max():.3f}")
+    assert rm.mean() < rd.mean()
+    assert rm.mean() <= 1.15
+    del torch
