@@ -372,10 +372,11 @@ int mpkx_batch_create(mpkx_batch **b, int nmat, const int *n,
                       const double *const *val, int nrhs, int nlevel,
                       double *sigmas);
 void mpkx_batch_destroy(mpkx_batch *b);
-/* Tuning configuration: block_width right-hand sides per pass (power of */
-/* two <= 16 dividing nrhs), lanes_per_row lanes splitting each row's    */
-/* entries (power of two, width*lanes <= 64), rows with more than        */
-/* long_row entries get a workgroup each.  Default (min(nrhs,8), 1, 512).*/
+/* Tuning configuration: block_width right-hand sides per pass, held in */
+/* each lane's registers (power of two <= 16 dividing nrhs),            */
+/* lanes_per_row lanes splitting each row's entries (power of two       */
+/* <= 64), rows with more than long_row entries get a workgroup each.   */
+/* Default (min(nrhs,8), 1, 512).                                       */
 int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row,
                          int long_row);
 /* X: device FP32 rows*nrhs; VV: device FP32 rows*nrhs*(nlevel+1) (may  */

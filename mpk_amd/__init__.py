@@ -148,6 +148,15 @@ def lib():
     return L
 
 
+def _release(name, handle):
+    """Destroys a library object; quiet at interpreter shutdown, when the
+    module's globals may already be gone."""
+    try:
+        getattr(lib(), name)(handle)
+    except (TypeError, AttributeError, OSError):
+        pass
+
+
 def _check(rc):
     if rc != 0:
         raise MpkxError(rc, lib().mpkx_last_error().decode())
@@ -396,7 +405,7 @@ class BufferSet:
             self._arr = None
             self.ranks = []
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         try:
             self.free()
         except Exception:
@@ -473,9 +482,9 @@ class Plan:
         _check(lib().mpkx_plan_stats(self._h, C.byref(r), C.byref(z), C.byref(g), C.byref(l)))
         return {"rows": r.value, "nnz": z.value, "ghosts": g.value, "launches": l.value}
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_plan_destroy(self._h)
+            _release('mpkx_plan_destroy', self._h)
             self._h = None
 
 
@@ -510,9 +519,9 @@ class PdBuffers:
         _check(lib().mpkx_pd_info(self._h, C.byref(npart), C.byref(nb), C.byref(mb), C.byref(mc)))
         return {"npart": npart.value, "batches": nb.value, "mbuf": mb.value, "mcol": mc.value}
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_pd_destroy(self._h)
+            _release('mpkx_pd_destroy', self._h)
             self._h = None
 
 
@@ -531,9 +540,9 @@ class PdPlan:
         _check(lib().mpkx_pd_plan_stats(self._h, C.byref(nb), C.byref(r), C.byref(z), C.byref(g)))
         return {"batches": nb.value, "rows": r.value, "nnz": z.value, "ghosts": g.value}
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_pd_plan_destroy(self._h)
+            _release('mpkx_pd_plan_destroy', self._h)
             self._h = None
 
 
@@ -551,9 +560,9 @@ class TaskPlan:
     def exec(self, x, vv, stream=None):
         _check(lib().mpkx_tasks_exec(self._h, _dev(x), _dev(vv), _stream(stream)))
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_tasks_destroy(self._h)
+            _release('mpkx_tasks_destroy', self._h)
             self._h = None
 
 
@@ -570,9 +579,9 @@ class CsrMatrix:
     def spmv_k(self, vv, k, stream=None, use_graph=True):
         _check(lib().mpkx_spmv_k(self._h, _dev(vv), k, _stream(stream), int(use_graph)))
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_csr_destroy(self._h)
+            _release('mpkx_csr_destroy', self._h)
             self._h = None
 
 
@@ -608,9 +617,9 @@ class FusedMPK:
                 "matrix_bytes": mb.value, "codec": ["explicit", "dict"][codec.value],
                 "launches": la.value}
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_fused_destroy(self._h)
+            _release('mpkx_fused_destroy', self._h)
             self._h = None
 
 
@@ -664,9 +673,9 @@ class FusedMPK32:
                 "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16"][vals.value],
                 "launches": la.value, "sigma": self.sigma}
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_fused32_destroy(self._h)
+            _release('mpkx_fused32_destroy', self._h)
             self._h = None
 
 
@@ -751,9 +760,9 @@ class BatchMPK:
         return {"rows": r.value, "nnz": z.value, "long_rows": lr.value, "launches": la.value,
                 "config": self.config}
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_batch_destroy(self._h)
+            _release('mpkx_batch_destroy', self._h)
             self._h = None
 
 
@@ -810,7 +819,7 @@ class Mlp:
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0, stream=None):
         _check(lib().mpkx_mlp_adam(self._h, lr, beta1, beta2, eps, weight_decay, _stream(stream)))
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         if getattr(self, "_h", None):
-            lib().mpkx_mlp_destroy(self._h)
+            _release('mpkx_mlp_destroy', self._h)
             self._h = None

@@ -1,8 +1,9 @@
 """Autotuning and the learned cost model for the batched block-RHS MPK.
 
 The tuning configuration of mpkx_batch is (B, L, T): B right-hand sides per
-pass over the matrix (vector block width), L lanes splitting each row's
-entries, and the long-row threshold T above which a row gets a workgroup.
+pass over the matrix, held in each lane's registers (vector block width), L
+lanes splitting each row's entries, and the long-row threshold T above which
+a row gets a workgroup.
 
 * `space(nrhs)` lists every valid configuration; `tune` times all of them on
   the GPU (the exhaustive autotuner, which is also the judge of any pick).
@@ -26,7 +27,7 @@ import numpy as np
 import mpk_amd as M
 
 WIDTHS = (1, 2, 4, 8, 16)
-LANES = (1, 2, 4, 8, 16, 32, 64)
+LANES = (1, 2, 4, 8, 16, 32)
 LONG_ROWS = (32, 128, 512, 2048)
 DEFAULT = (8, 1, 512)  # mpkx_batch_create's configuration (B = min(nrhs, 8))
 TAU = 0.03
@@ -39,7 +40,7 @@ def default_config(nrhs):
 def space(nrhs):
     """Every valid (B, L, T) for nrhs right-hand sides."""
     return [(b, l, t) for b in WIDTHS if b <= nrhs and nrhs % b == 0
-            for l in LANES if b * l <= 64 for t in LONG_ROWS]
+            for l in LANES for t in LONG_ROWS]
 
 
 def time_exec(bm, X, VV, reps=3, warmup=1):
@@ -239,6 +240,16 @@ def train(feats, timings, steps=4000, lr=2e-3, weight_decay=1e-4, seed=0, log=No
     return model, hist
 
 
+def best_fixed(timings, nrhs_list):
+    """For each right-hand-side count, the one configuration with the lowest
+    mean regret over the given samples (the best tuning without features)."""
+    out = {}
+    for r in sorted(set(nrhs_list)):
+        ts = [t for t, q in zip(timings, nrhs_list) if q == r]
+        out[r] = min(space(r), key=lambda c: float(np.mean(regret(ts, [c] * len(ts)))))
+    return out
+
+
 def regret(timings, picks):
     """time(pick) / time(best) for each sample."""
     return np.array([t[p] / min(t.values()) for t, p in zip(timings, picks)])
@@ -301,17 +312,20 @@ def main(argv=None):
     nh = int(len(idx) * a.holdout)
     hold, tr = idx[:nh], idx[nh:]
     model, hist = train(feats[tr], [timings[i] for i in tr], steps=a.steps, log=log)
-    out = {}
+    fixed = best_fixed([timings[i] for i in tr], [specs[i]["nrhs"] for i in tr])
+    out = {"best_fixed_configs": {str(k): list(v) for k, v in fixed.items()}}
     for name, sel in (("train", tr), ("holdout", hold)):
         if len(sel) == 0:
             continue
         tm = [timings[i] for i in sel]
         picks = [nearest(p, specs[i]["nrhs"]) for p, i in zip(model.predict(feats[sel]), sel)]
         defaults = [default_config(specs[i]["nrhs"]) for i in sel]
-        rm, rd = regret(tm, picks), regret(tm, defaults)
+        fixed_picks = [fixed[specs[i]["nrhs"]] for i in sel]
+        rm, rd, rf = regret(tm, picks), regret(tm, defaults), regret(tm, fixed_picks)
         out[name] = {"n": int(len(sel)), "model_mean_regret": float(rm.mean()),
                      "model_max_regret": float(rm.max()), "default_mean_regret": float(rd.mean()),
-                     "default_max_regret": float(rd.max()),
+                     "default_max_regret": float(rd.max()), "best_fixed_mean_regret": float(rf.mean()),
+                     "best_fixed_max_regret": float(rf.max()),
                      "model_within_5pct": float(np.mean(rm <= 1.05))}
         log(f"{name}: {out[name]}")
     model.meta.update({"evaluation": out, "loss_history": hist, "data": [str(p) for p in a.data]})

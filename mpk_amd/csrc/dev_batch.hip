@@ -9,16 +9,20 @@
 //
 // Power-law hubs make skirt tiling useless (the distance-1 ball of a hub is
 // a large part of its graph), so every power is one block SpMM:
-//   group rows: B*L lanes per row; lane (l, r) owns right-hand side c0 + r
-//               and entries rp[row] + l, + L, ...; the L partial sums meet in
-//               a butterfly of lane shuffles.  The B lanes of one l read the
-//               same (col, val) and one contiguous B*4-byte piece of x[col].
-//   long rows:  (nnz > T) a 256-lane workgroup per row: 256/B groups stride
-//               the row and reduce in LDS.
+//   group rows: L lanes per row; lane l sums entries rp[row] + l, + L, ...
+//               for B right-hand sides held in its registers (x[col] read as
+//               16-byte vectors); the L partial sums meet in a butterfly of
+//               lane shuffles.  One (col, val) load serves B products.
+//   long rows:  (nnz > T) a 256-lane workgroup per row; the lanes stride the
+//               row and reduce in LDS.
 //   passes:     R/B passes over the matrix per power when B < R.
+//   XCD order:  workgroup i takes row block xcd_remap(i), so each of the 8
+//               XCDs walks a contiguous range of rows (whole matrices) and
+//               a matrix's x segment stays in that XCD's L2.
 // (B, L, T) is the tuning configuration.  A row's operation order depends
-// only on its own entries and (B, L, T), never on the other matrices, so a
-// matrix's result is bitwise the same batched or alone.
+// only on its own entries and L (or the workgroup width), never on the
+// other matrices or on B, so a matrix's result is bitwise the same batched
+// or alone, and every column equals a one-vector run.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -32,48 +36,104 @@ namespace {
 
 constexpr int BT = 256;
 
+// B consecutive right-hand sides of one x row, as B/4 16-byte loads (or one
+// 8- / 4-byte load for B = 2 / 1).
+template <int B>
+__device__ __forceinline__ void load_rhs(const float *p, float (&v)[B]) {
+  if constexpr (B % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < B / 4; q++) {
+      const float4 t = reinterpret_cast<const float4 *>(p)[q];
+      v[4 * q] = t.x, v[4 * q + 1] = t.y, v[4 * q + 2] = t.z, v[4 * q + 3] = t.w;
+    }
+  } else if constexpr (B == 2) {
+    const float2 t = *reinterpret_cast<const float2 *>(p);
+    v[0] = t.x, v[1] = t.y;
+  } else {
+    v[0] = *p;
+  }
+}
+
+template <int B>
+__device__ __forceinline__ void store_rhs(float *p, const float (&v)[B]) {
+  if constexpr (B % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < B / 4; q++)
+      reinterpret_cast<float4 *>(p)[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  } else if constexpr (B == 2) {
+    *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+  } else {
+    *p = v[0];
+  }
+}
+
+// L lanes per row; lane l sums entries rp[row] + l, + L, ... for B
+// right-hand sides held in registers; the L partial sums meet in a
+// butterfly of lane shuffles.
 template <int B, int L>
 __global__ __launch_bounds__(BT) void k_spmm_group(const int *__restrict__ rows, int nrows,
                                                    const int *__restrict__ rp, const int *__restrict__ ci,
                                                    const float *__restrict__ va, const float *__restrict__ x,
                                                    float *__restrict__ y, int ldv, int c0) {
-  constexpr int G = B * L;
-  static_assert(G <= 64 && (G & (G - 1)) == 0, "row group must fit a wavefront");
-  const long g = (long)blockIdx.x * BT + threadIdx.x;
-  const long slot = g / G;
-  const int t = (int)(g % G), r = t % B, l = t / B;
+  static_assert(L <= 64 && (L & (L - 1)) == 0, "row group must fit a wavefront");
+  // consecutive rows (one matrix, one x segment) stay on one XCD's L2
+  const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * BT + threadIdx.x;
+  const long slot = g / L;
+  const int l = (int)(g % L);
   const bool active = slot < nrows;
   const int row = active ? rows[slot] : 0;
-  float acc = 0.0f;
+  float acc[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) acc[b] = 0.0f;
   if (active) {
     const int e = rp[row + 1];
-    const float *xc = x + c0 + r;
-    for (int j = rp[row] + l; j < e; j += L) acc = fmaf(va[j], xc[(size_t)ci[j] * ldv], acc);
+    const float *xc = x + c0;
+    for (int j = rp[row] + l; j < e; j += L) {
+      const float a = va[j];
+      float xv[B];
+      load_rhs<B>(xc + (size_t)ci[j] * ldv, xv);
+#pragma unroll
+      for (int b = 0; b < B; b++) acc[b] = fmaf(a, xv[b], acc[b]);
+    }
   }
 #pragma unroll
-  for (int off = G / 2; off >= B; off >>= 1) acc += __shfl_xor(acc, off, G);
-  if (active && l == 0) y[(size_t)row * ldv + c0 + r] = acc;
+  for (int off = L / 2; off >= 1; off >>= 1)
+#pragma unroll
+    for (int b = 0; b < B; b++) acc[b] += __shfl_xor(acc[b], off, L);
+  if (active && l == 0) store_rhs<B>(y + (size_t)row * ldv + c0, acc);
 }
 
+// Rows longer than T: one 256-lane workgroup per row, lane t sums entries
+// t, t + 256, ...; the partial sums are combined by a pairwise tree in LDS.
 template <int B>
 __global__ __launch_bounds__(BT) void k_spmm_long(const int *__restrict__ rows, const int *__restrict__ rp,
                                                   const int *__restrict__ ci, const float *__restrict__ va,
                                                   const float *__restrict__ x, float *__restrict__ y, int ldv,
                                                   int c0) {
-  constexpr int G = BT / B;
-  __shared__ float red[BT];
-  const int row = rows[blockIdx.x];
-  const int grp = threadIdx.x / B, r = threadIdx.x % B;
-  const float *xc = x + c0 + r;
-  float acc = 0.0f;
-  for (int j = rp[row] + grp; j < rp[row + 1]; j += G) acc = fmaf(va[j], xc[(size_t)ci[j] * ldv], acc);
-  red[threadIdx.x] = acc;
+  __shared__ float red[B][BT];
+  const int row = rows[xcd_remap(blockIdx.x, gridDim.x)];
+  const int t = threadIdx.x;
+  float acc[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) acc[b] = 0.0f;
+  const float *xc = x + c0;
+  for (int j = rp[row] + t; j < rp[row + 1]; j += BT) {
+    const float a = va[j];
+    float xv[B];
+    load_rhs<B>(xc + (size_t)ci[j] * ldv, xv);
+#pragma unroll
+    for (int b = 0; b < B; b++) acc[b] = fmaf(a, xv[b], acc[b]);
+  }
+#pragma unroll
+  for (int b = 0; b < B; b++) red[b][t] = acc[b];
   __syncthreads();
-  for (int w = G / 2; w > 0; w >>= 1) {
-    if (grp < w) red[threadIdx.x] += red[threadIdx.x + w * B];
+  for (int w = BT / 2; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int b = 0; b < B; b++) red[b][t] += red[b][t + w];
     __syncthreads();
   }
-  if (grp == 0) y[(size_t)row * ldv + c0 + r] = red[r];
+  if (t < B) y[(size_t)row * ldv + c0 + t] = red[t][0];
 }
 
 struct LevelArgs {
@@ -84,7 +144,7 @@ struct LevelArgs {
 
 template <int B, int L>
 void launch_group(const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
-  long threads = (long)a.nshort * B * L;
+  long threads = (long)a.nshort * L;
   hipLaunchKernelGGL((k_spmm_group<B, L>), dim3((unsigned)((threads + BT - 1) / BT)), dim3(BT), 0, st, a.srows,
                      a.nshort, a.rp, a.ci, a.va, x, y, a.ldv, c0);
 }
@@ -95,10 +155,10 @@ bool launch_group_l(int L, const LevelArgs &a, const float *x, float *y, int c0,
     case 1: launch_group<B, 1>(a, x, y, c0, st); return true;
     case 2: launch_group<B, 2>(a, x, y, c0, st); return true;
     case 4: launch_group<B, 4>(a, x, y, c0, st); return true;
-    case 8: if constexpr (B * 8 <= 64) { launch_group<B, 8>(a, x, y, c0, st); return true; } break;
-    case 16: if constexpr (B * 16 <= 64) { launch_group<B, 16>(a, x, y, c0, st); return true; } break;
-    case 32: if constexpr (B * 32 <= 64) { launch_group<B, 32>(a, x, y, c0, st); return true; } break;
-    case 64: if constexpr (B * 64 <= 64) { launch_group<B, 64>(a, x, y, c0, st); return true; } break;
+    case 8: launch_group<B, 8>(a, x, y, c0, st); return true;
+    case 16: launch_group<B, 16>(a, x, y, c0, st); return true;
+    case 32: launch_group<B, 32>(a, x, y, c0, st); return true;
+    case 64: launch_group<B, 64>(a, x, y, c0, st); return true;
   }
   return false;
 }
@@ -154,9 +214,8 @@ int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row, int 
   if (!pow2(block_width) || block_width > 16 || b->nrhs % block_width != 0)
     return fail(MPKX_EUNSUPPORTED, "batch_configure: block width %d (power of two <= 16 dividing nrhs %d)",
                 block_width, b->nrhs);
-  if (!pow2(lanes_per_row) || block_width * lanes_per_row > 64)
-    return fail(MPKX_EUNSUPPORTED, "batch_configure: lanes per row %d (power of two, width*lanes <= 64)",
-                lanes_per_row);
+  if (!pow2(lanes_per_row) || lanes_per_row > 64)
+    return fail(MPKX_EUNSUPPORTED, "batch_configure: lanes per row %d (power of two <= 64)", lanes_per_row);
   if (long_row < 1) return fail(MPKX_EINVAL, "batch_configure: long-row threshold %d", long_row);
   std::vector<int> srows, lrows;
   try {

@@ -73,7 +73,7 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   f->fn = pick64(P.T, P.MR, P.W, P.dict);
   if (!f->fn) return fail(MPKX_EUNSUPPORTED, "fused_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   f->lds_bytes = (size_t)2 * P.maxslots * sizeof(double);
-  f->slots = (long)P.grow.size();
+  f->slots = P.region_slots;
   f->matrix_bytes = (long)P.ent.size() * 4 + (long)P.vals.size() * 8;
   if (P.vals.empty()) P.vals.push_back(0.0);
   if ((rc = f->hdr.upload(P.hdr)) || (rc = f->grow.upload(P.grow)) || (rc = f->ent.upload(P.ent)) ||

@@ -121,7 +121,7 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
     if (!f->fnf) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   }
   f->lds_bytes = (size_t)2 * P.maxslots * sizeof(float);
-  f->slots = (long)P.grow.size();
+  f->slots = P.region_slots;
   std::vector<float> vf;
   std::vector<uint16_t> vb;
   if (f->values == 1) {

@@ -28,22 +28,32 @@
 //           (dictionary of <= 4 distinct values, passed as kernel arguments),
 //           or the slot alone with a value stream [row][w] (FP64,
 //           FP32 or BF16).  Padding entries are 0xffff and skipped.
+//   sharing the row list is stored relative to the tile's first row (the
+//           header holds that base), and row lists, ELL blocks and value
+//           streams are stored once per distinct content: the interior
+//           tiles of a regular grid all read the same few KB from L2
+//           instead of their own copies from HBM.
 #pragma once
 
 #include <algorithm>
 #include <cstring>
 #include <memory>
 #include <type_traits>
+#include <unordered_map>
+#include <vector>
 
 #include "dev_common.h"
 
 namespace mpkx {
 
 constexpr int FUSED_HDR = 16;       // ints per tile header
+// header: [0] row-list offset, [1] ELL offset, [2] value offset, [3] region
+// rows, [4 .. 4+s] cnt, [15] row base (added to every stored row number)
 constexpr int FUSED_MAX_S = 10;     // levels per launch (header holds cnt[0..s])
 constexpr uint32_t FUSED_PAD16 = 0xffffu;
 constexpr int FUSED_MAX_SLOTS = 16383;  // 14-bit slot field
 constexpr long FUSED_LDS_BYTES = 160 * 1024;
+static_assert(FUSED_HDR >= 4 + FUSED_MAX_S + 2, "header holds cnt[0..s] and the row base");
 
 // value of a stored matrix entry
 __device__ __forceinline__ double mval_of(double v) { return v; }
@@ -74,6 +84,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   const size_t eo = (size_t)(unsigned)h[1];
   const size_t vo = (size_t)(unsigned)h[2];
   const int *cnt = h + 4;
+  const int base = h[FUSED_HDR - 1];
   const int tid = threadIdx.x;
 
   // gather: GB slots per thread in flight at a time (row indices, then
@@ -85,7 +96,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     int gi[GB];
     VT gv[GB];
 #pragma unroll
-    for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] : -1;
+    for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] + base : -1;
 #pragma unroll
     for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
 #pragma unroll
@@ -120,7 +131,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
 #pragma unroll
       for (int k = 0; k < W2; k++) e[m][k] = 0xffffffffu;
     }
-    own[m] = r < nown ? g[r] : -1;
+    own[m] = r < nown ? g[r] + base : -1;
   }
   __syncthreads();
 
@@ -202,7 +213,27 @@ struct TilePlan {
   std::vector<uint32_t> ent;
   std::vector<double> vals;   // value stream (explicit codec), [m][k][T] per tile
   long row_levels = 0;
+  long region_slots = 0;  // sum of the tiles' region rows (before sharing)
 };
+
+// Content-addressed append: stores seg once; returns its offset in store.
+template <class V>
+long share_segment(std::vector<V> &store, std::unordered_multimap<uint64_t, long> &index, const std::vector<V> &seg) {
+  uint64_t h = 1469598103934665603ull ^ seg.size();
+  const unsigned char *b = reinterpret_cast<const unsigned char *>(seg.data());
+  for (size_t i = 0; i < seg.size() * sizeof(V); i++) h = (h ^ b[i]) * 1099511628211ull;
+  auto range = index.equal_range(h);
+  for (auto it = range.first; it != range.second; ++it) {
+    long off = it->second;
+    if (off + (long)seg.size() <= (long)store.size() &&
+        memcmp(store.data() + off, seg.data(), seg.size() * sizeof(V)) == 0)
+      return off;
+  }
+  long off = (long)store.size();
+  store.insert(store.end(), seg.begin(), seg.end());
+  index.emplace(h, off);
+  return off;
+}
 
 // Builds the plan.  slot_bytes: LDS bytes per region row (2 buffers).
 // mr_choices: allowed (T, MR) pairs, smallest first.
@@ -326,21 +357,20 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.ent.clear();
   P.vals.clear();
   const int W2 = (W + 1) / 2;
-  long slot_off = 0, ent_off = 0, val_off = 0;
+  std::unordered_multimap<uint64_t, long> gidx, eidx, vidx;
+  std::vector<int> tmpl;
+  std::vector<uint32_t> eseg;
+  std::vector<double> vseg;
+  P.region_slots = 0;
   for (int p = 0; p < npart; p++) {
     const Tile &t = tiles[p];
     int ncomp = t.cnt[s - 1];
     int mrt = (ncomp + T - 1) / T;
-    int *h = &P.hdr[(size_t)p * FUSED_HDR];
-    h[0] = (int)slot_off;
-    h[1] = (int)ent_off;
-    h[2] = (int)val_off;
-    h[3] = (int)t.slots.size();
-    for (int d = 0; d <= s; d++) h[4 + d] = t.cnt[d];
-    P.grow.insert(P.grow.end(), t.slots.begin(), t.slots.end());
-    size_t eseg = (size_t)mrt * W2 * T, vseg = use_dict ? 0 : (size_t)mrt * W * T;
-    P.ent.resize(ent_off + eseg, 0xffffffffu);
-    P.vals.resize(val_off + vseg, 0.0);
+    const int base = t.slots[0];
+    tmpl.resize(t.slots.size());
+    for (size_t k = 0; k < t.slots.size(); k++) tmpl[k] = t.slots[k] - base;
+    eseg.assign((size_t)mrt * W2 * T, 0xffffffffu);
+    vseg.assign(use_dict ? 0 : (size_t)mrt * W * T, 0.0);
     for (int r = 0; r < ncomp; r++) {
       for (int k = 0; k < W; k++) {
         int sl = t.rows[(size_t)r * W + k];
@@ -351,18 +381,27 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
           for (size_t c = 0; c < dict.size(); c++)
             if (memcmp(&dict[c], &v, sizeof(double)) == 0) code = (uint32_t)c;
         } else {
-          P.vals[val_off + (size_t)r * W + k] = v;
+          vseg[(size_t)r * W + k] = v;
         }
         uint32_t e16 = (uint32_t)sl | (code << 14);
-        uint32_t &word = P.ent[ent_off + (size_t)r * W2 + k / 2];
+        uint32_t &word = eseg[(size_t)r * W2 + k / 2];
         word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
       }
     }
-    slot_off += (long)t.slots.size();
-    ent_off += (long)eseg;
-    val_off += (long)vseg;
-    if (ent_off > INT32_MAX || slot_off > INT32_MAX || val_off > INT32_MAX)
+    long slot_off = share_segment(P.grow, gidx, tmpl);
+    long ent_off = share_segment(P.ent, eidx, eseg);
+    long val_off = use_dict ? 0 : share_segment(P.vals, vidx, vseg);
+    if (ent_off + (long)eseg.size() > INT32_MAX || slot_off + (long)tmpl.size() > INT32_MAX ||
+        val_off + (long)vseg.size() > INT32_MAX)
       return fail(MPKX_EUNSUPPORTED, "fused: plan exceeds 32-bit offsets");
+    int *h = &P.hdr[(size_t)p * FUSED_HDR];
+    h[0] = (int)slot_off;
+    h[1] = (int)ent_off;
+    h[2] = (int)val_off;
+    h[3] = (int)t.slots.size();
+    for (int d = 0; d <= s; d++) h[4 + d] = t.cnt[d];
+    h[FUSED_HDR - 1] = base;
+    P.region_slots += (long)t.slots.size();
   }
   long rl = 0;
   for (int L0 = 0; L0 < nlevel; L0 += s) {

@@ -1059,8 +1059,9 @@ ORC_API int orc_pd_exec(const char *prefix, int np, const double *x, double *y) 
 /* Row order of operations: rows with nnz <= T are split over L lanes  */
 /* (lane l sums entries l, l+L, ... with fmaf from 0), the L partial   */
 /* sums combined by a butterfly (for m = L/2 .. 1: a_l += a_{l xor m});*/
-/* rows with nnz > T over G = 256/B groups the same way, the group     */
-/* sums combined by a tree (for w = G/2 .. 1: a_g += a_{g+w}, g < w).  */
+/* rows with nnz > T over 256 lanes the same way, the lane sums        */
+/* combined by a tree (for w = 128 .. 1: a_g += a_{g+w}, g < w).  Each */
+/* right-hand side is summed on its own, so B does not enter the order.*/
 /* X, VV row-major [row][nrhs]; VV level-major with X as level 0.      */
 /* ------------------------------------------------------------------ */
 ORC_API double orc_pow2_sigma(int n, const int *ptr, const double *val) {
@@ -1115,7 +1116,7 @@ ORC_API int orc_batch_mpk32(int n, const int *ptr, const int *col, const double 
       int big = ptr[i + 1] - ptr[i] > long_row;
       for (int c = 0; c < nrhs; c++)
         y[(long)i * nrhs + c] = orc_row32(col, v, ptr[i], ptr[i + 1], x, nrhs, c,
-                                          big ? 256 / block_width : lanes);
+                                          big ? 256 : lanes);
     }
   }
   free(v);

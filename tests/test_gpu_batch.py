@@ -44,7 +44,8 @@ def _split(VV, bm, m):
     return VV.reshape(bm.nlevel + 1, bm.rows, bm.nrhs)[:, r0:r1, :]
 
 
-@pytest.mark.parametrize("cfg", [(8, 1, 256), (4, 4, 32), (2, 8, 128), (1, 64, 2048), (8, 8, 16), (2, 2, 32)])
+@pytest.mark.parametrize("cfg", [(8, 1, 256), (4, 4, 32), (2, 8, 128), (1, 64, 2048), (8, 8, 16), (2, 2, 32),
+                                 (8, 32, 64)])
 def test_batch_bitwise_vs_oracle(cfg):
     say("batch vs oracle", cfg)
     mats = _mats(3, 7)
@@ -80,17 +81,17 @@ def test_batched_equals_unbatched():
 
 
 def test_batch_columns_equal_single_vector():
-    """Short rows only (T above every row): column c of an 8-RHS run equals the
-    1-RHS run on column c."""
+    """Every right-hand side is summed on its own: column c of an 8-RHS run
+    equals the 1-RHS run on column c (short and long rows)."""
     say("columns vs single vector")
     mats = _mats(2, 3, mean=6.0)
     k = 4
     bm8 = M.BatchMPK(mats, 8, k)
-    bm8.configure(8, 2, 1 << 30)
+    bm8.configure(8, 2, 32)
     X = _x(bm8.rows, 8, 3)
     VV8 = _run(bm8, X).reshape(k + 1, bm8.rows, 8)
     bm1 = M.BatchMPK(mats, 1, k)
-    bm1.configure(1, 2, 1 << 30)
+    bm1.configure(1, 2, 32)
     for c in range(8):
         VV1 = _run(bm1, np.ascontiguousarray(X.reshape(-1, 8)[:, c]))
         assert np.array_equal(VV8[:, :, c].ravel(), VV1), c
@@ -147,7 +148,9 @@ def test_batch_stats_and_errors():
     with pytest.raises(M.MpkxError):
         bm.configure(16, 1, 64)  # wider than nrhs
     with pytest.raises(M.MpkxError):
-        bm.configure(8, 16, 64)  # 128 lanes per row
+        bm.configure(8, 128, 64)  # more lanes per row than a wavefront
+    with pytest.raises(M.MpkxError):
+        bm.configure(8, 3, 64)
     with pytest.raises(M.MpkxError):
         M.BatchMPK(mats, 3, 2)
 

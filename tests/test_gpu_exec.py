@@ -101,6 +101,28 @@ def test_fused_explicit_values_bitwise_oracle(typ, size, blocks, k, s):
     assert np.array_equal(to_host(vv), gold)
 
 
+@pytest.mark.parametrize("blocks,s", [((4, 4, 4), 2), ((2, 4, 8), 3)])
+def test_fused_shared_segments_bitwise_oracle(blocks, s):
+    """Box tiles of a grid fall into at most 3 classes per dimension (low
+    boundary, interior, high boundary), so the plan stores at most 27
+    distinct ELL blocks and row lists; results stay bitwise."""
+    say("fused shared segments", blocks, s)
+    rp, ci, va = M.gen2("cube", 48, -1.0)
+    n = len(rp) - 1
+    part = M.partition_grid([48, 48, 48], list(blocks))
+    k = 6
+    x = M.uniform_pm1(5, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    F = M.FusedMPK(rp, ci, va, part, k, s)
+    st = F.stats()
+    classes = int(np.prod([min(b, 3) for b in blocks]))
+    words = (7 + 1) // 2
+    assert st["matrix_bytes"] <= classes * (st["max_slots"] + 1024) * words * 4, st
+    vv = zeros_dev(n * (k + 1))
+    F.exec(to_dev(x), vv)
+    assert np.array_equal(to_host(vv), gold)
+
+
 def test_fused_graph_partition_bitwise_oracle():
     say("fused graph partition")
     rp, ci, va = M.gen2("cube", 24, -1.0)
