@@ -376,7 +376,8 @@ void mpkx_batch_destroy(mpkx_batch *b);
 /* each lane's registers (power of two <= 16 dividing nrhs),            */
 /* lanes_per_row lanes splitting each row's entries (power of two       */
 /* <= 64), rows with more than long_row entries get a workgroup each.   */
-/* Default (min(nrhs,8), 1, 512).                                       */
+/* Default (min(nrhs,16), 4, 32): the best single configuration over  */
+/* the synthetic tuning suite.                                          */
 int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row,
                          int long_row);
 /* X: device FP32 rows*nrhs; VV: device FP32 rows*nrhs*(nlevel+1) (may  */

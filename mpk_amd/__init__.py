@@ -745,7 +745,7 @@ class BatchMPK:
                                        _p(self.sigmas)))
         self.row_offsets = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
         self.rows = int(self.row_offsets[-1])
-        self.config = (min(nrhs, 8), 1, 512)
+        self.config = (min(nrhs, 16), 4, 32)
 
     def configure(self, block_width, lanes_per_row, long_row):
         _check(lib().mpkx_batch_configure(self._h, block_width, lanes_per_row, long_row))

@@ -29,7 +29,7 @@ import mpk_amd as M
 WIDTHS = (1, 2, 4, 8, 16)
 LANES = (1, 2, 4, 8, 16, 32)
 LONG_ROWS = (32, 128, 512, 2048)
-DEFAULT = (8, 1, 512)  # mpkx_batch_create's configuration (B = min(nrhs, 8))
+DEFAULT = (16, 4, 32)  # mpkx_batch_create's configuration (B = min(nrhs, 16))
 TAU = 0.03
 
 

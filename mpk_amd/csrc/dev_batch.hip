@@ -309,7 +309,7 @@ int mpkx_batch_create(mpkx_batch **out, int nmat, const int *ns, const int *cons
     if (bad[m]) return fail(MPKX_EINVAL, "batch_create: matrix %d has %s", m, bad[m] == 1 ? "decreasing ptr" : "a column out of range");
   int rc;
   if ((rc = b->rp.upload(b->rp_host)) || (rc = b->ci.upload(ci)) || (rc = b->va.upload(va))) return rc;
-  if ((rc = mpkx_batch_configure(b, nrhs < 8 ? nrhs : 8, 1, 512))) return rc;
+  if ((rc = mpkx_batch_configure(b, nrhs < 16 ? nrhs : 16, 4, 32))) return rc;
   *out = guard.release();
   return MPKX_OK;
 }

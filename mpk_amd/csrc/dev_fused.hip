@@ -16,10 +16,10 @@ __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src,
 
 using K64 = FusedKernel<double, double>;
 
-K64 pick64(int T, int MR, int W, bool dict) {
-  if (T == 512 && MR == 4) return pick_fused_w<double, double, 512, 4>(W, dict);
-  if (T == 1024 && MR == 4) return pick_fused_w<double, double, 1024, 4>(W, dict);
-  if (T == 1024 && MR == 8) return pick_fused_w<double, double, 1024, 8>(W, dict);
+K64 pick64(int T, int MR, int W, int nd) {
+  if (T == 512 && MR == 4) return pick_fused_w<double, double, 512, 4>(W, nd);
+  if (T == 1024 && MR == 4) return pick_fused_w<double, double, 1024, 4>(W, nd);
+  if (T == 1024 && MR == 8) return pick_fused_w<double, double, 1024, 8>(W, nd);
   return nullptr;
 }
 
@@ -70,9 +70,9 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   if (rc) return rc;
   TilePlan &P = f->plan;
   for (size_t i = 0; i < P.dictv.size(); i++) f->dv[i] = P.dictv[i];
-  f->fn = pick64(P.T, P.MR, P.W, P.dict);
+  f->fn = pick64(P.T, P.MR, P.W, P.ndict);
   if (!f->fn) return fail(MPKX_EUNSUPPORTED, "fused_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
-  f->lds_bytes = (size_t)2 * P.maxslots * sizeof(double);
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double);  // + the zero slot
   f->slots = P.region_slots;
   f->matrix_bytes = (long)P.ent.size() * 4 + (long)P.vals.size() * 8;
   if (P.vals.empty()) P.vals.push_back(0.0);

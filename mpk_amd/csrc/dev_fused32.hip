@@ -23,11 +23,11 @@ using K32f = FusedKernel<float, float>;
 using K32b = FusedKernel<float, uint16_t>;
 
 template <typename MT>
-FusedKernel<float, MT> pick32(int T, int MR, int W, bool dict) {
-  if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4>(W, dict);
-  if (T == 1024 && MR == 4) return pick_fused_w<float, MT, 1024, 4>(W, dict);
-  if (T == 1024 && MR == 8) return pick_fused_w<float, MT, 1024, 8>(W, dict);
-  if (T == 1024 && MR == 16) return pick_fused_w<float, MT, 1024, 16>(W, dict);
+FusedKernel<float, MT> pick32(int T, int MR, int W, int nd) {
+  if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4>(W, nd);
+  if (T == 1024 && MR == 4) return pick_fused_w<float, MT, 1024, 4>(W, nd);
+  if (T == 1024 && MR == 8) return pick_fused_w<float, MT, 1024, 8>(W, nd);
+  if (T == 1024 && MR == 16) return pick_fused_w<float, MT, 1024, 16>(W, nd);
   return nullptr;
 }
 
@@ -114,13 +114,13 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
   f->inv_sigma = (float)(1.0 / f->sigma);
   if (sigma_out) *sigma_out = f->sigma;
   if (f->values == 2) {
-    f->fnb = pick32<uint16_t>(P.T, P.MR, P.W, false);
+    f->fnb = pick32<uint16_t>(P.T, P.MR, P.W, 0);
     if (!f->fnb) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   } else {
-    f->fnf = pick32<float>(P.T, P.MR, P.W, P.dict);
+    f->fnf = pick32<float>(P.T, P.MR, P.W, P.ndict);
     if (!f->fnf) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   }
-  f->lds_bytes = (size_t)2 * P.maxslots * sizeof(float);
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(float);  // + the zero slot
   f->slots = P.region_slots;
   std::vector<float> vf;
   std::vector<uint16_t> vb;

@@ -27,7 +27,10 @@
 //           thread lives in thread r % T's registers for the whole launch.  An entry is a 14-bit slot plus a 2-bit value code
 //           (dictionary of <= 4 distinct values, passed as kernel arguments),
 //           or the slot alone with a value stream [row][w] (FP64,
-//           FP32 or BF16).  Padding entries are 0xffff and skipped.
+//           FP32 or BF16).  Padding entries point to the zero slot (an
+//           extra LDS word that is always 0) with code 0 / value 0, so
+//           every row runs W branch-free multiply-adds: adding +-0 never
+//           changes a sum that started at +0, so results stay bitwise.
 //   sharing the row list is stored relative to the tile's first row (the
 //           header holds that base), and row lists, ELL blocks and value
 //           streams are stored once per distinct content: the interior
@@ -36,6 +39,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -50,8 +54,9 @@ constexpr int FUSED_HDR = 16;       // ints per tile header
 // header: [0] row-list offset, [1] ELL offset, [2] value offset, [3] region
 // rows, [4 .. 4+s] cnt, [15] row base (added to every stored row number)
 constexpr int FUSED_MAX_S = 10;     // levels per launch (header holds cnt[0..s])
-constexpr uint32_t FUSED_PAD16 = 0xffffu;
-constexpr int FUSED_MAX_SLOTS = 16383;  // 14-bit slot field
+// 14-bit slot field; slot maxslots (<= 16382) is the zero slot: it holds
+// 0 in both LDS buffers and padding entries point to it
+constexpr int FUSED_MAX_SLOTS = 16382;
 constexpr long FUSED_LDS_BYTES = 160 * 1024;
 static_assert(FUSED_HDR >= 4 + FUSED_MAX_S + 2, "header holds cnt[0..s] and the row base");
 
@@ -67,9 +72,10 @@ __device__ __forceinline__ double mac(double s, double a, double x) { return __d
 __device__ __forceinline__ float mac(float s, float a, float x) { return fmaf(a, x, s); }
 
 // VT: vector / accumulation type (double or float).  MT: stored matrix
-// value type when DICT is false (double, float or uint16_t = bf16).
-// inv_sigma scales every new power (power of two: exact); FP64 passes 1.
-template <typename VT, typename MT, int T, int MR, int W, bool DICT>
+// value type when ND == 0 (double, float or uint16_t = bf16).  ND: number
+// of dictionary values (1, 2 or 4; 0 = value stream).  inv_sigma scales
+// every new power (power of two: exact); FP64 passes 1.
+template <typename VT, typename MT, int T, int MR, int W, int ND>
 __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
                                              const uint32_t *__restrict__ ent, const MT *__restrict__ vals,
                                              VT d0, VT d1, VT d2, VT d3, VT inv_sigma, VT *vv, int n,
@@ -77,7 +83,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   constexpr int W2 = (W + 1) / 2;
   constexpr bool SCALE = std::is_same<VT, float>::value;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + maxslots;
+  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + maxslots + 1;
   const int tile = xcd_remap(blockIdx.x, ntiles);
   const int *h = hdr + (size_t)tile * FUSED_HDR;
   const int *g = grow + h[0];
@@ -103,6 +109,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     for (int u = 0; u < GB; u++)
       if (gi[u] >= 0) buf0[t0 + u * T] = gv[u];
   }
+  if (tid == 0) buf0[maxslots] = buf1[maxslots] = (VT)0;
 
   // matrix rows of this thread into registers (W2 consecutive words per
   // row: one wide load), own-row targets
@@ -158,17 +165,20 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
         VT s = 0;
 #pragma unroll
         for (int k = 0; k < W; k++) {
-          const uint32_t x = (k & 1) ? (ew[k >> 1] >> 16) : (ew[k >> 1] & 0xffffu);
-          if (x != FUSED_PAD16) {
-            VT a;
-            if (DICT) {
-              const uint32_t c = x >> 14;
-              a = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
-            } else {
-              a = (VT)mval_of(vp[vo + (size_t)r * W + k]);
-            }
-            s = mac(s, a, in[x & 0x3fffu]);
+          const uint32_t wv = ew[k >> 1];
+          const uint32_t sl = (k & 1) ? (wv >> 16) & 0x3fffu : wv & 0x3fffu;
+          VT a;
+          if constexpr (ND == 0) {
+            a = (VT)mval_of(vp[vo + (size_t)r * W + k]);
+          } else if constexpr (ND == 1) {
+            a = d0;
+          } else if constexpr (ND == 2) {
+            a = (wv & ((k & 1) ? 0x40000000u : 0x4000u)) ? d1 : d0;
+          } else {
+            const uint32_t c = (k & 1) ? wv >> 30 : (wv >> 14) & 3u;
+            a = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
           }
+          s = mac(s, a, in[sl]);
         }
         if (SCALE) s = s * inv_sigma;
         out[r] = s;
@@ -189,18 +199,28 @@ template <typename VT, typename MT>
 using FusedKernel = void (*)(const int *, const int *, const uint32_t *, const MT *, VT, VT, VT, VT, VT,
                              VT *, int, int, int, int, int);
 
-template <typename VT, typename MT, int T, int MR>
-FusedKernel<VT, MT> pick_fused_w(int W, bool dict) {
-#define MPKX_FUSED_CASE(WW) \
-  case WW:                  \
-    return dict ? k_fused<VT, MT, T, MR, WW, true> : k_fused<VT, MT, T, MR, WW, false>;
-  switch (W) {
-    MPKX_FUSED_CASE(5)
-    MPKX_FUSED_CASE(7)
-    MPKX_FUSED_CASE(9)
-    MPKX_FUSED_CASE(16)
+template <typename VT, typename MT, int T, int MR, int W>
+FusedKernel<VT, MT> pick_fused_nd(int nd) {
+  if constexpr (std::is_same<MT, uint16_t>::value)  // BF16 values always stream
+    return nd == 0 ? k_fused<VT, MT, T, MR, W, 0> : nullptr;
+  switch (nd) {
+    case 0: return k_fused<VT, MT, T, MR, W, 0>;
+    case 1: return k_fused<VT, MT, T, MR, W, 1>;
+    case 2: return k_fused<VT, MT, T, MR, W, 2>;
+    case 4: return k_fused<VT, MT, T, MR, W, 4>;
   }
-#undef MPKX_FUSED_CASE
+  return nullptr;
+}
+
+// nd: dictionary size (1, 2, 4) or 0 for a value stream.
+template <typename VT, typename MT, int T, int MR>
+FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
+  switch (W) {
+    case 5: return pick_fused_nd<VT, MT, T, MR, 5>(nd);
+    case 7: return pick_fused_nd<VT, MT, T, MR, 7>(nd);
+    case 9: return pick_fused_nd<VT, MT, T, MR, 9>(nd);
+    case 16: return pick_fused_nd<VT, MT, T, MR, 16>(nd);
+  }
   return nullptr;
 }
 
@@ -208,6 +228,7 @@ FusedKernel<VT, MT> pick_fused_w(int W, bool dict) {
 struct TilePlan {
   int n = 0, nlevel = 0, s = 0, ntiles = 0, W = 0, T = 0, MR = 0, maxslots = 0;
   bool dict = false;
+  int ndict = 0;              // kernel dictionary size: 1, 2 or 4 (0: value stream)
   std::vector<double> dictv;  // up to 4 distinct values
   std::vector<int> hdr, grow;
   std::vector<uint32_t> ent;
@@ -264,6 +285,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
         dict.push_back(val[j]);
     }
   }
+  for (double d : dict)  // padding multiplies code 0 by the zero slot: keep it finite
+    if (!std::isfinite(d)) use_dict = false;
   if (!use_dict) dict.clear();
 
   std::vector<std::vector<int>> members(npart);
@@ -329,10 +352,10 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   }
   if (maxslots > FUSED_MAX_SLOTS)
     return fail(MPKX_EUNSUPPORTED, "fused: %d region rows exceed %d slots", maxslots, FUSED_MAX_SLOTS);
-  if ((long)maxslots * slot_bytes > FUSED_LDS_BYTES)
+  if ((long)(maxslots + 1) * slot_bytes > FUSED_LDS_BYTES)
     return fail(MPKX_EUNSUPPORTED,
                 "fused: tile region of %d rows needs %ld B LDS (> %ld); use more partitions or a smaller s",
-                maxslots, (long)maxslots * slot_bytes, FUSED_LDS_BYTES);
+                maxslots, (long)(maxslots + 1) * slot_bytes, FUSED_LDS_BYTES);
   int T = 0, MR = 0;
   for (auto &c : mr_choices)  // register budget: at most 64 packed words per thread
     if (maxcomp <= c.first * c.second && c.second * ((W + 1) / 2) <= 64) {
@@ -352,6 +375,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.maxslots = maxslots;
   P.dict = use_dict;
   P.dictv = dict;
+  P.ndict = !use_dict ? 0 : dict.size() <= 1 ? 1 : dict.size() == 2 ? 2 : 4;
+  const uint32_t zero_slot = (uint32_t)maxslots;
   P.hdr.assign((size_t)npart * FUSED_HDR, 0);
   P.grow.clear();
   P.ent.clear();
@@ -374,7 +399,11 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
     for (int r = 0; r < ncomp; r++) {
       for (int k = 0; k < W; k++) {
         int sl = t.rows[(size_t)r * W + k];
-        if (sl < 0) continue;
+        if (sl < 0) {  // padding: zero slot, code 0, value 0
+          uint32_t &word = eseg[(size_t)r * W2 + k / 2];
+          word = (k & 1) ? ((word & 0x0000ffffu) | (zero_slot << 16)) : ((word & 0xffff0000u) | zero_slot);
+          continue;
+        }
         uint32_t code = 0;
         double v = t.vals[(size_t)r * W + k];
         if (use_dict) {

@@ -213,7 +213,7 @@ def test_config_space_and_targets():
     times = {c: 1.0 for c in CM.space(8)}
     times[(4, 2, 128)] = 0.5
     np.testing.assert_allclose(CM.soft_target(times), np.log2([4, 2, 128]), atol=1e-6)
-    assert CM.default_config(4) == (4, 1, 512)
+    assert CM.default_config(4) == (4, 4, 32)
     assert all(CM.default_config(r) in CM.space(r) for r in (1, 2, 4, 8, 16))
 
 
