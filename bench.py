@@ -125,7 +125,7 @@ def measured_traffic(N, k, s, blocks):
     return None
 
 
-def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C):
+def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C, config=None):
     """256 Chung-Lu matrices (rows log-uniform in [1k, 200k], exponent 2.5,
     about 10 entries per row, N(0,1) values, diagonal = row abs-sum + 1), FP32,
     8 right-hand sides each, k = 6, all powers of all matrices in one hipGraph
@@ -145,9 +145,9 @@ def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C):
     X = torch.rand(rows * nrhs, device=dev, dtype=torch.float32) * 2 - 1
     VV = torch.empty(rows * nrhs * (k + 1), device=dev, dtype=torch.float32)
     t0 = time.perf_counter()
-    times = CM.tune(bm, X, VV, reps=3)
+    times = CM.tune(bm, X, VV, reps=3, configs=[tuple(config), CM.default_config(nrhs)] if config else None)
     t_tune = time.perf_counter() - t0
-    best = min(times, key=times.get)
+    best = tuple(config) if config else min(times, key=times.get)
     bm.configure(*best)
     ms = CM.time_exec(bm, X, VV, reps=steps, warmup=2)
     # error of the smallest matrix against FP64 (scipy) per power, one right-hand side
@@ -168,7 +168,8 @@ def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C):
     return {"workload": f"{count} Chung-Lu matrices, rows {rows}, nnz {nnz}, {nrhs} RHS, k={k}, FP32",
             "ms_per_step": round(ms, 5), "value": round(gbs, 2), "unit": "GB/s",
             "pct_hbm_roofline": round(100 * gbs / HBM_PEAK_GBS, 2),
-            "bytes_per_power": bpp, "config": list(best), "default_config_ms": round(times[CM.default_config(nrhs)], 5),
+            "bytes_per_power": bpp, "config": list(best), "autotuned": config is None,
+            "default_config_ms": round(times[CM.default_config(nrhs)], 5),
             "configs_tried": len(times), "tune_seconds": round(t_tune, 2), "generate_seconds": round(t_gen, 2),
             "launches_in_graph": bm.stats()["launches"], "long_rows": bm.stats()["long_rows"],
             "per_power_rel_err_vs_fp64_smallest_matrix": errs}
@@ -205,6 +206,35 @@ def learned_tuning(M, torch, dev, path, n=1500000, nrhs=16, k=10, seed=0xC5):
             "model_rank": f"{rank}/{len(times)}", "in_training_range": model.in_range(f),
             "model_seconds": round(t_model, 4), "exhaustive_seconds": round(t_sweep, 2),
             "trained_on": model.meta.get("samples")}
+
+
+def sstep_cg(M, torch, dev, rp, ci, va, N, blocks, rtol=1e-8):
+    """The consumer: s-step CG (A/sigma, monomial basis) on the same 128^3
+    Laplacian, b = U[-1,1) (seed 7), x0 = 0, to rtol; s = 1 (classical CG,
+    one SpMV per step) against s = 4 (two fused 4-power MPKs per 4 steps)."""
+    import scipy.sparse as sp
+    n = len(rp) - 1
+    b = torch.from_numpy(M.uniform_pm1(7, n)).to(dev)
+    part = M.partition_grid([N, N, N], blocks)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    out = {"workload": f"CA-CG on the {N}^3 7-pt Laplacian, rtol {rtol:g}"}
+    for s in (1, 4):
+        S = M.SStepCG(rp, ci, va, part, s, min(s, 2))
+        x = torch.zeros(n, dtype=torch.float64, device=dev)
+        S.solve(b, x, rtol=rtol)  # warm-up (plan graphs, caches)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, est = S.solve(b, x, rtol=rtol)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        xh = x.cpu().numpy()
+        bh = b.cpu().numpy()
+        true = float(np.linalg.norm(bh - A @ xh) / np.linalg.norm(bh))
+        out[f"s{s}"] = {"iterations": it, "ms": round(dt * 1e3, 3), "us_per_step": round(dt * 1e6 / it, 2),
+                        "residual_estimate": float("%.3e" % est), "true_residual": float("%.3e" % true)}
+        del S
+    out["speedup_s4_vs_s1"] = round(out["s1"]["ms"] / out["s4"]["ms"], 3)
+    return out
 
 
 def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
@@ -248,6 +278,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-matrix form")
+    ap.add_argument("--no-sstep", action="store_true", help="skip the s-step CG consumer")
+    ap.add_argument("--batch-config", default=None,
+                    help="B,L,T for the batched form instead of the exhaustive autotuner (profiling runs)")
     ap.add_argument("--no-tuning", action="store_true", help="skip the learned-tuning evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -417,9 +450,16 @@ def main():
     host_vv = vf.cpu().numpy()
     del F, A, vf, vp
     torch.cuda.empty_cache()
+    if not args.no_sstep:
+        try:
+            out["sstep_cg"] = sstep_cg(M, torch, dev, rp, ci, va, N, blocks)
+        except Exception as e:
+            out["sstep_cg"] = {"error": str(e)}
+        torch.cuda.empty_cache()
     if not args.no_batch:
         try:
-            out["batched_form"] = batched_form(M, torch, dev)
+            cfg = [int(v) for v in args.batch_config.split(",")] if args.batch_config else None
+            out["batched_form"] = batched_form(M, torch, dev, config=cfg)
         except Exception as e:
             out["batched_form"] = {"error": str(e)}
         torch.cuda.empty_cache()

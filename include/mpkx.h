@@ -414,6 +414,25 @@ int mpkx_mlp_loss_grad(mpkx_mlp *m, int N, const float *X, const float *T,
 int mpkx_mlp_adam(mpkx_mlp *m, float lr, float beta1, float beta2, float eps,
                   float weight_decay, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* s-step conjugate gradients (CA-CG, monomial basis) on the fused MPK:  */
+/* the solver the Krylov block is built for (the reference only sketches */
+/* it, jupyter/s-step-cg.ipynb).  A must be SPD; the solver works on     */
+/* A/sigma, b/sigma (sigma = power of two >= max row abs-sum, exact).    */
+/* part/npart: fused-MPK tiles; s: CG steps per outer iteration (1..8); */
+/* levels_per_launch: the fused kernel's s.                             */
+/* ------------------------------------------------------------------ */
+typedef struct mpkx_sstep mpkx_sstep;
+int mpkx_sstep_create(mpkx_sstep **S, int n, const int *ptr, const int *col,
+                      const double *val, const int *part, int npart, int s,
+                      int levels_per_launch);
+void mpkx_sstep_destroy(mpkx_sstep *S);
+/* b, x device (n); x0 = 0.  Stops when ||r|| / ||b|| <= rtol (from the  */
+/* Gram matrix) or after max_outer outer iterations; *iters = CG steps   */
+/* taken (outer * s), *resid = the final relative residual estimate.     */
+int mpkx_sstep_solve(mpkx_sstep *S, const double *b, double *x, double rtol,
+                     int max_outer, int *iters, double *resid, void *stream);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

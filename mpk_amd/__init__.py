@@ -139,6 +139,9 @@ def lib():
         "mpkx_mlp_forward": ([P, C.c_int, P, P, P], C.c_int),
         "mpkx_mlp_loss_grad": ([P, C.c_int, P, P, P, P], C.c_int),
         "mpkx_mlp_adam": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, P], C.c_int),
+        "mpkx_sstep_create": ([P, C.c_int, P, P, P, P, C.c_int, C.c_int, C.c_int], C.c_int),
+        "mpkx_sstep_destroy": ([P], None),
+        "mpkx_sstep_solve": ([P, P, P, C.c_double, C.c_int, P, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -822,4 +825,30 @@ class Mlp:
     def __del__(self, _release=_release):
         if getattr(self, "_h", None):
             _release('mpkx_mlp_destroy', self._h)
+            self._h = None
+
+
+# ---- s-step CG on the fused MPK ---------------------------------------------------
+
+class SStepCG:
+    """CA-CG (monomial basis, s steps per outer iteration) driven by the fused
+    MPK; A must be SPD.  solve(b, x) works on device arrays, x0 = 0."""
+
+    def __init__(self, ptr, col, val, part, s, levels_per_launch=2):
+        self._h = C.c_void_p()
+        part = _i32(part)
+        self.s = s
+        _check(lib().mpkx_sstep_create(C.byref(self._h), len(ptr) - 1, _p(_i32(ptr)), _p(_i32(col)),
+                                       _p(np.ascontiguousarray(val, np.float64)), _p(part),
+                                       int(part.max()) + 1, s, levels_per_launch))
+
+    def solve(self, b, x, rtol=1e-8, max_outer=10000, stream=None):
+        it, res = C.c_int(), C.c_double()
+        _check(lib().mpkx_sstep_solve(self._h, _dev(b), _dev(x), rtol, max_outer, C.byref(it), C.byref(res),
+                                      _stream(stream)))
+        return it.value, res.value
+
+    def __del__(self, _release=_release):
+        if getattr(self, "_h", None):
+            _release('mpkx_sstep_destroy', self._h)
             self._h = None

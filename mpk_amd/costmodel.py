@@ -128,9 +128,10 @@ def generate(specs, pool, window):
         yield f.result()
 
 
-def collect(specs, nlevel=2, reps=3, threads=8, log=None):
+def collect(specs, nlevel=2, reps=3, threads=8, log=None, checkpoint=None):
     """Features and exhaustive timings of every spec (GPU).  Returns
-    (features [N,24], list of {config: ms})."""
+    (features [N,24], list of {config: ms}).  checkpoint(specs_done, feats,
+    timings) is called every 100 matrices."""
     import torch
     feats, timings = [], []
     t0 = time.time()
@@ -146,6 +147,8 @@ def collect(specs, nlevel=2, reps=3, threads=8, log=None):
             del bm
             if log and (i + 1) % 25 == 0:
                 log(f"collect {i + 1}/{len(specs)} {time.time() - t0:.0f}s")
+            if checkpoint and (i + 1) % 100 == 0:
+                checkpoint(specs[:i + 1], np.array(feats), timings)
     return np.array(feats), timings
 
 
@@ -297,7 +300,7 @@ def main(argv=None):
 
     if a.cmd == "collect":
         specs = suite(a.count, a.seed)[a.start:]
-        feats, timings = collect(specs, log=log)
+        feats, timings = collect(specs, log=log, checkpoint=lambda sp, f, t: save_data(a.out, sp, f, t))
         save_data(a.out, specs, feats, timings)
         log(f"wrote {a.out}: {len(specs)} matrices")
         return

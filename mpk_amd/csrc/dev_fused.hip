@@ -72,7 +72,7 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   for (size_t i = 0; i < P.dictv.size(); i++) f->dv[i] = P.dictv[i];
   f->fn = pick64(P.T, P.MR, P.W, P.ndict);
   if (!f->fn) return fail(MPKX_EUNSUPPORTED, "fused_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double);  // + the zero slot
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double) * (P.ndict ? P.ndict : 1);  // + the zero record
   f->slots = P.region_slots;
   f->matrix_bytes = (long)P.ent.size() * 4 + (long)P.vals.size() * 8;
   if (P.vals.empty()) P.vals.push_back(0.0);

@@ -24,10 +24,14 @@
 //   matrix  compute rows (d <= s-1) as an ELL block of W entries per row,
 //           two 16-bit entries per 32-bit word, the W2 words of a row
 //           consecutive [row][word] (one wide load per row); row r = m*T +
-//           thread lives in thread r % T's registers for the whole launch.  An entry is a 14-bit slot plus a 2-bit value code
-//           (dictionary of <= 4 distinct values, passed as kernel arguments),
-//           or the slot alone with a value stream [row][w] (FP64,
-//           FP32 or BF16).  Padding entries point to the zero slot (an
+//           thread lives in thread r % T's registers for the whole launch.
+//           With a dictionary of ND <= 4 distinct values (kernel arguments)
+//           the LDS holds, for every slot, the ND products d_c * x (each
+//           rounded as the reference rounds a_ij * x_j), and an entry is the
+//           LDS index slot * ND + c of its product: a row is W loads and W
+//           adds, the products are formed once per slot by its writer.
+//           Otherwise an entry is the slot and the values stream as
+//           [row][w] (FP64, FP32 or BF16).  Padding entries point to the zero slot (an
 //           extra LDS word that is always 0) with code 0 / value 0, so
 //           every row runs W branch-free multiply-adds: adding +-0 never
 //           changes a sum that started at +0, so results stay bitwise.
@@ -54,8 +58,9 @@ constexpr int FUSED_HDR = 16;       // ints per tile header
 // header: [0] row-list offset, [1] ELL offset, [2] value offset, [3] region
 // rows, [4 .. 4+s] cnt, [15] row base (added to every stored row number)
 constexpr int FUSED_MAX_S = 10;     // levels per launch (header holds cnt[0..s])
-// 14-bit slot field; slot maxslots (<= 16382) is the zero slot: it holds
-// 0 in both LDS buffers and padding entries point to it
+// region rows per tile: an entry is a 16-bit LDS index slot * ND + c
+// (ND <= 4), and slot maxslots is the zero record: it holds 0 (or its
+// products) in both LDS buffers and padding entries point to it
 constexpr int FUSED_MAX_SLOTS = 16382;
 constexpr long FUSED_LDS_BYTES = 160 * 1024;
 static_assert(FUSED_HDR >= 4 + FUSED_MAX_S + 2, "header holds cnt[0..s] and the row base");
@@ -71,6 +76,31 @@ __device__ __forceinline__ float mval_of(uint16_t v) { return __uint_as_float((u
 __device__ __forceinline__ double mac(double s, double a, double x) { return __dadd_rn(s, __dmul_rn(a, x)); }
 __device__ __forceinline__ float mac(float s, float a, float x) { return fmaf(a, x, s); }
 
+// product and sum in the accumulation precision (FP64: separately rounded,
+// the reference's operation sequence)
+__device__ __forceinline__ double mul(double a, double x) { return __dmul_rn(a, x); }
+__device__ __forceinline__ float mul(float a, float x) { return a * x; }
+__device__ __forceinline__ double add(double s, double p) { return __dadd_rn(s, p); }
+__device__ __forceinline__ float add(float s, float p) { return s + p; }
+
+// LDS record of one slot: the value (ND == 0) or its ND dictionary products.
+template <typename VT, int ND>
+__device__ __forceinline__ void put_slot(VT *buf, int slot, VT v, VT d0, VT d1, VT d2, VT d3) {
+  if constexpr (ND == 0) {
+    buf[slot] = v;
+  } else if constexpr (ND == 1) {
+    buf[slot] = mul(d0, v);
+  } else {
+    VT *q = buf + (size_t)slot * ND;
+    q[0] = mul(d0, v);
+    q[1] = mul(d1, v);
+    if constexpr (ND == 4) {
+      q[2] = mul(d2, v);
+      q[3] = mul(d3, v);
+    }
+  }
+}
+
 // VT: vector / accumulation type (double or float).  MT: stored matrix
 // value type when ND == 0 (double, float or uint16_t = bf16).  ND: number
 // of dictionary values (1, 2 or 4; 0 = value stream).  inv_sigma scales
@@ -82,8 +112,9 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
                                              int L0, int w, int ntiles, int maxslots) {
   constexpr int W2 = (W + 1) / 2;
   constexpr bool SCALE = std::is_same<VT, float>::value;
+  constexpr int NP = ND > 0 ? ND : 1;  // LDS words per slot
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + maxslots + 1;
+  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + (size_t)(maxslots + 1) * NP;
   const int tile = xcd_remap(blockIdx.x, ntiles);
   const int *h = hdr + (size_t)tile * FUSED_HDR;
   const int *g = grow + h[0];
@@ -107,9 +138,12 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
 #pragma unroll
     for (int u = 0; u < GB; u++)
-      if (gi[u] >= 0) buf0[t0 + u * T] = gv[u];
+      if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, gv[u], d0, d1, d2, d3);
   }
-  if (tid == 0) buf0[maxslots] = buf1[maxslots] = (VT)0;
+  if (tid == 0) {
+    put_slot<VT, ND>(buf0, maxslots, (VT)0, d0, d1, d2, d3);
+    put_slot<VT, ND>(buf1, maxslots, (VT)0, d0, d1, d2, d3);
+  }
 
   // matrix rows of this thread into registers (W2 consecutive words per
   // row: one wide load), own-row targets
@@ -166,22 +200,14 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
 #pragma unroll
         for (int k = 0; k < W; k++) {
           const uint32_t wv = ew[k >> 1];
-          const uint32_t sl = (k & 1) ? (wv >> 16) & 0x3fffu : wv & 0x3fffu;
-          VT a;
-          if constexpr (ND == 0) {
-            a = (VT)mval_of(vp[vo + (size_t)r * W + k]);
-          } else if constexpr (ND == 1) {
-            a = d0;
-          } else if constexpr (ND == 2) {
-            a = (wv & ((k & 1) ? 0x40000000u : 0x4000u)) ? d1 : d0;
-          } else {
-            const uint32_t c = (k & 1) ? wv >> 30 : (wv >> 14) & 3u;
-            a = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
-          }
-          s = mac(s, a, in[sl]);
+          const uint32_t idx = (k & 1) ? wv >> 16 : wv & 0xffffu;
+          if constexpr (ND == 0)
+            s = mac(s, (VT)mval_of(vp[vo + (size_t)r * W + k]), in[idx]);
+          else
+            s = add(s, in[idx]);
         }
         if (SCALE) s = s * inv_sigma;
-        out[r] = s;
+        put_slot<VT, ND>(out, r, s, d0, d1, d2, d3);
         if (own[m] >= 0) dst[own[m]] = s;
       }
       // keep the next row's LDS loads below this point: one row's operands
@@ -352,6 +378,11 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   }
   if (maxslots > FUSED_MAX_SLOTS)
     return fail(MPKX_EUNSUPPORTED, "fused: %d region rows exceed %d slots", maxslots, FUSED_MAX_SLOTS);
+  const int ndk = !use_dict ? 1 : dict.size() <= 1 ? 1 : dict.size() == 2 ? 2 : 4;
+  if (use_dict && ndk > 1 && (long)(maxslots + 1) * slot_bytes * ndk > FUSED_LDS_BYTES) {
+    use_dict = false;  // the product records do not fit: stream the values
+    dict.clear();
+  }
   if ((long)(maxslots + 1) * slot_bytes > FUSED_LDS_BYTES)
     return fail(MPKX_EUNSUPPORTED,
                 "fused: tile region of %d rows needs %ld B LDS (> %ld); use more partitions or a smaller s",
@@ -376,7 +407,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.dict = use_dict;
   P.dictv = dict;
   P.ndict = !use_dict ? 0 : dict.size() <= 1 ? 1 : dict.size() == 2 ? 2 : 4;
-  const uint32_t zero_slot = (uint32_t)maxslots;
+  const uint32_t np = P.ndict ? (uint32_t)P.ndict : 1u;  // LDS words per slot
+  const uint32_t zero_slot = (uint32_t)maxslots * np;  // LDS index of the zero record (code 0)
   P.hdr.assign((size_t)npart * FUSED_HDR, 0);
   P.grow.clear();
   P.ent.clear();
@@ -412,7 +444,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
         } else {
           vseg[(size_t)r * W + k] = v;
         }
-        uint32_t e16 = (uint32_t)sl | (code << 14);
+        uint32_t e16 = (uint32_t)sl * np + code;  // LDS index of the value or of the product d_code * x
         uint32_t &word = eseg[(size_t)r * W2 + k / 2];
         word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
       }

@@ -1196,3 +1196,43 @@ ORC_API double orc_mlp_loss_grad(int N, const double *X, const double *T, const 
   }
   return loss;
 }
+
+/* ------------------------------------------------------------------ */
+/* Classical conjugate gradients in FP64 (Hestenes-Stiefel), x0 = 0,   */
+/* the checker of the s-step solver (the reference notebook            */
+/* jupyter/s-step-cg.ipynb starts from the same iteration).  Stops at  */
+/* ||r|| <= rtol ||b||; returns the iteration count.                   */
+/* ------------------------------------------------------------------ */
+ORC_API int orc_cg(int n, const int *ptr, const int *col, const double *val, const double *b,
+                   double rtol, int maxit, double *x) {
+  double *r = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * n);
+  double rr = 0.0, bb;
+  for (int i = 0; i < n; i++) {
+    x[i] = 0.0;
+    r[i] = p[i] = b[i];
+    rr += b[i] * b[i];
+  }
+  bb = rr;
+  int it = 0;
+  while (it < maxit && rr > rtol * rtol * bb) {
+    double pq = 0.0;
+    for (int i = 0; i < n; i++) {
+      double s = 0.0;
+      for (int j = ptr[i]; j < ptr[i + 1]; j++) s += val[j] * p[col[j]];
+      q[i] = s;
+      pq += p[i] * s;
+    }
+    double alpha = rr / pq, rr1 = 0.0;
+    for (int i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      rr1 += r[i] * r[i];
+    }
+    double beta = rr1 / rr;
+    for (int i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+    rr = rr1;
+    it++;
+  }
+  free(r); free(p); free(q);
+  return it;
+}

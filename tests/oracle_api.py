@@ -50,6 +50,8 @@ def lib():
         L.orc_mlp_nparam.restype = C.c_long
         L.orc_mlp_loss_grad.argtypes = [C.c_int, P, P, P, P, P]
         L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_cg.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_int, P]
+        L.orc_cg.restype = C.c_int
         _lib = L
     return _lib
 
@@ -261,3 +263,12 @@ def mlp_loss_grad(X, T, params):
     g = np.zeros(mlp_nparam())
     loss = lib().orc_mlp_loss_grad(X.shape[0], ptr(X), ptr(T), ptr(params), ptr(y), ptr(g))
     return loss, y, g
+
+
+def cg(rp, ci, va, b, rtol, maxit=100000):
+    """Classical FP64 CG from x0 = 0; returns (x, iterations)."""
+    x = np.zeros(len(rp) - 1)
+    it = lib().orc_cg(len(rp) - 1, ptr(np.ascontiguousarray(rp, np.int32)), ptr(np.ascontiguousarray(ci, np.int32)),
+                      ptr(np.ascontiguousarray(va, np.float64)), ptr(np.ascontiguousarray(b, np.float64)),
+                      rtol, maxit, ptr(x))
+    return x, it

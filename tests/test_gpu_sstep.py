@@ -1,0 +1,50 @@
+"""GPU tests of the s-step conjugate-gradient solver built on the fused MPK,
+checked against the oracle's classical FP64 CG (same start x0 = 0)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mpk_amd as M
+import tests.oracle_api as O
+from tests.gpu_util import say, to_dev, to_host, zeros_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(N=32):
+    rp, ci, va = M.gen2("cube", N, -1.0)  # SPD 7-point Laplacian, diag 6
+    b = M.uniform_pm1(7, len(rp) - 1)
+    return rp, ci, va, b
+
+
+@pytest.mark.parametrize("s,lpl", [(1, 1), (2, 2), (4, 2), (4, 4)])
+def test_sstep_cg_matches_classical_cg(s, lpl):
+    say("s-step CG", s, lpl)
+    rp, ci, va, b = _problem()
+    n = len(rp) - 1
+    rtol = 1e-8
+    x_cg, it_cg = O.cg(rp, ci, va, b, rtol)
+    part = M.partition_grid([32, 32, 32], [2, 4, 4])
+    S = M.SStepCG(rp, ci, va, part, s, lpl)
+    x = zeros_dev(n)
+    it, res = S.solve(to_dev(b), x, rtol=rtol)
+    xh = to_host(x)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    true_res = np.linalg.norm(b - A @ xh) / np.linalg.norm(b)
+    say(f"classical CG {it_cg} its; s={s}: {it} its, estimate {res:.2e}, true {true_res:.2e}")
+    assert res <= rtol
+    assert true_res <= 10 * rtol
+    assert np.linalg.norm(xh - x_cg) / np.linalg.norm(x_cg) <= 1e-6
+    if s == 1:  # CA-CG with s = 1 is classical CG
+        assert abs(it - it_cg) <= 1
+    assert it_cg - s <= it <= 1.25 * it_cg + s
+
+
+def test_sstep_cg_errors():
+    rp, ci, va, b = _problem(8)
+    part = np.zeros(len(rp) - 1, np.int32)
+    with pytest.raises(M.MpkxError):
+        M.SStepCG(rp, ci, va, part, 9)
+    S = M.SStepCG(rp, ci, va, part, 2)
+    with pytest.raises(M.MpkxError):
+        S.solve(to_dev(b), zeros_dev(len(b)), rtol=0.0)
