@@ -267,8 +267,9 @@ int mpkx_fused_exec(mpkx_fused *f, const double *x, double *vv, void *stream,
  * (vv[L0*n ..] must already hold level L0).  For per-kernel timing. */
 int mpkx_fused_round(mpkx_fused *f, double *vv, int round, void *stream);
 /* Plan statistics: max slots per tile, total slots, computed row-levels
- * per exec, matrix bytes, codec (0 explicit FP64, 1 dictionary),
- * launches per exec. */
+ * per exec, matrix bytes, codec (0 explicit FP64, 1 dictionary with
+ * product records in LDS, 2 dictionary with per-entry select), launches
+ * per exec. */
 int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots,
                      long *row_levels, long *matrix_bytes, int *codec,
                      int *launches);

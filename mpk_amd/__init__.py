@@ -617,7 +617,8 @@ class FusedMPK:
         _check(lib().mpkx_fused_stats(self._h, C.byref(ms), C.byref(sl), C.byref(rl), C.byref(mb),
                                       C.byref(codec), C.byref(la)))
         return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
-                "matrix_bytes": mb.value, "codec": ["explicit", "dict"][codec.value],
+                "matrix_bytes": mb.value, "codec": ["explicit", "dict", "dict"][codec.value],
+                "dict_codec": [None, "products", "select"][codec.value],
                 "launches": la.value}
 
     def __del__(self, _release=_release):

@@ -72,7 +72,7 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   for (size_t i = 0; i < P.dictv.size(); i++) f->dv[i] = P.dictv[i];
   f->fn = pick64(P.T, P.MR, P.W, P.ndict);
   if (!f->fn) return fail(MPKX_EUNSUPPORTED, "fused_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double) * (P.ndict ? P.ndict : 1);  // + the zero record
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double) * (P.ndict > 0 ? P.ndict : 1);  // + the zero record
   f->slots = P.region_slots;
   f->matrix_bytes = (long)P.ent.size() * 4 + (long)P.vals.size() * 8;
   if (P.vals.empty()) P.vals.push_back(0.0);
@@ -122,7 +122,7 @@ int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots, long *ro
   if (slots) *slots = f->slots;
   if (row_levels) *row_levels = f->plan.row_levels;
   if (matrix_bytes) *matrix_bytes = f->matrix_bytes;
-  if (codec) *codec = f->plan.dict ? 1 : 0;
+  if (codec) *codec = !f->plan.dict ? 0 : f->plan.ndict > 0 ? 1 : 2;
   if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
   return MPKX_OK;
 }

@@ -120,7 +120,7 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
     f->fnf = pick32<float>(P.T, P.MR, P.W, P.ndict);
     if (!f->fnf) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   }
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(float) * (P.ndict ? P.ndict : 1);  // + the zero record
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(float) * (P.ndict > 0 ? P.ndict : 1);  // + the zero record
   f->slots = P.region_slots;
   std::vector<float> vf;
   std::vector<uint16_t> vb;

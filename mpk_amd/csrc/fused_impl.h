@@ -26,10 +26,14 @@
 //           consecutive [row][word] (one wide load per row); row r = m*T +
 //           thread lives in thread r % T's registers for the whole launch.
 //           With a dictionary of ND <= 4 distinct values (kernel arguments)
-//           the LDS holds, for every slot, the ND products d_c * x (each
-//           rounded as the reference rounds a_ij * x_j), and an entry is the
-//           LDS index slot * ND + c of its product: a row is W loads and W
-//           adds, the products are formed once per slot by its writer.
+//           there are two codecs.  Product records: the LDS holds, for every
+//           slot, the ND products d_c * x (each rounded as the reference
+//           rounds a_ij * x_j), and an entry is the LDS index slot * ND + c
+//           of its product: a row is W loads and W adds, the products are
+//           formed once per slot by its writer.  Select: the LDS holds x,
+//           an entry is the slot plus a 2-bit code in bits 14-15, and the
+//           value is selected from the dictionary per entry (less LDS, more
+//           VALU; chosen when product records would cost occupancy).
 //           Otherwise an entry is the slot and the values stream as
 //           [row][w] (FP64, FP32 or BF16).  Padding entries point to the zero slot (an
 //           extra LDS word that is always 0) with code 0 / value 0, so
@@ -83,10 +87,10 @@ __device__ __forceinline__ float mul(float a, float x) { return a * x; }
 __device__ __forceinline__ double add(double s, double p) { return __dadd_rn(s, p); }
 __device__ __forceinline__ float add(float s, float p) { return s + p; }
 
-// LDS record of one slot: the value (ND == 0) or its ND dictionary products.
+// LDS record of one slot: the value (ND <= 0) or its ND dictionary products.
 template <typename VT, int ND>
 __device__ __forceinline__ void put_slot(VT *buf, int slot, VT v, VT d0, VT d1, VT d2, VT d3) {
-  if constexpr (ND == 0) {
+  if constexpr (ND <= 0) {
     buf[slot] = v;
   } else if constexpr (ND == 1) {
     buf[slot] = mul(d0, v);
@@ -102,8 +106,9 @@ __device__ __forceinline__ void put_slot(VT *buf, int slot, VT v, VT d0, VT d1, 
 }
 
 // VT: vector / accumulation type (double or float).  MT: stored matrix
-// value type when ND == 0 (double, float or uint16_t = bf16).  ND: number
-// of dictionary values (1, 2 or 4; 0 = value stream).  inv_sigma scales
+// value type when ND == 0 (double, float or uint16_t = bf16).  ND: 0 =
+// value stream; 1, 2, 4 = product records of that many dictionary values;
+// -2, -4 = select codec with 2 or 4 dictionary values.  inv_sigma scales
 // every new power (power of two: exact); FP64 passes 1.
 template <typename VT, typename MT, int T, int MR, int W, int ND>
 __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
@@ -201,10 +206,20 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
         for (int k = 0; k < W; k++) {
           const uint32_t wv = ew[k >> 1];
           const uint32_t idx = (k & 1) ? wv >> 16 : wv & 0xffffu;
-          if constexpr (ND == 0)
+          if constexpr (ND == 0) {
             s = mac(s, (VT)mval_of(vp[vo + (size_t)r * W + k]), in[idx]);
-          else
+          } else if constexpr (ND > 0) {
             s = add(s, in[idx]);
+          } else {
+            VT a;
+            if constexpr (ND == -2) {
+              a = (idx & 0x4000u) ? d1 : d0;
+            } else {
+              const uint32_t c = idx >> 14;
+              a = c == 0 ? d0 : c == 1 ? d1 : c == 2 ? d2 : d3;
+            }
+            s = mac(s, a, in[idx & 0x3fffu]);
+          }
         }
         if (SCALE) s = s * inv_sigma;
         put_slot<VT, ND>(out, r, s, d0, d1, d2, d3);
@@ -234,11 +249,13 @@ FusedKernel<VT, MT> pick_fused_nd(int nd) {
     case 1: return k_fused<VT, MT, T, MR, W, 1>;
     case 2: return k_fused<VT, MT, T, MR, W, 2>;
     case 4: return k_fused<VT, MT, T, MR, W, 4>;
+    case -2: return k_fused<VT, MT, T, MR, W, -2>;
+    case -4: return k_fused<VT, MT, T, MR, W, -4>;
   }
   return nullptr;
 }
 
-// nd: dictionary size (1, 2, 4) or 0 for a value stream.
+// nd: 0 value stream, 1/2/4 product records, -2/-4 select codec.
 template <typename VT, typename MT, int T, int MR>
 FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
   switch (W) {
@@ -254,7 +271,7 @@ FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
 struct TilePlan {
   int n = 0, nlevel = 0, s = 0, ntiles = 0, W = 0, T = 0, MR = 0, maxslots = 0;
   bool dict = false;
-  int ndict = 0;              // kernel dictionary size: 1, 2 or 4 (0: value stream)
+  int ndict = 0;              // kernel codec: 0 value stream, 1/2/4 product records, -2/-4 select
   std::vector<double> dictv;  // up to 4 distinct values
   std::vector<int> hdr, grow;
   std::vector<uint32_t> ent;
@@ -378,11 +395,6 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   }
   if (maxslots > FUSED_MAX_SLOTS)
     return fail(MPKX_EUNSUPPORTED, "fused: %d region rows exceed %d slots", maxslots, FUSED_MAX_SLOTS);
-  const int ndk = !use_dict ? 1 : dict.size() <= 1 ? 1 : dict.size() == 2 ? 2 : 4;
-  if (use_dict && ndk > 1 && (long)(maxslots + 1) * slot_bytes * ndk > FUSED_LDS_BYTES) {
-    use_dict = false;  // the product records do not fit: stream the values
-    dict.clear();
-  }
   if ((long)(maxslots + 1) * slot_bytes > FUSED_LDS_BYTES)
     return fail(MPKX_EUNSUPPORTED,
                 "fused: tile region of %d rows needs %ld B LDS (> %ld); use more partitions or a smaller s",
@@ -395,6 +407,19 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
       break;
     }
   if (!T) return fail(MPKX_EUNSUPPORTED, "fused: %d compute rows per tile exceed the register blocking", maxcomp);
+  // Dictionary codec: product records (ndk LDS words per slot) unless they
+  // lower the number of resident workgroups per CU (LDS or 32 waves), in
+  // which case the select codec keeps one word per slot.
+  const int ndk = !use_dict ? 1 : dict.size() <= 1 ? 1 : dict.size() == 2 ? 2 : 4;
+  bool products = use_dict;
+  if (use_dict && ndk > 1) {
+    auto resident = [&](int words) {
+      long lds = (long)(maxslots + 1) * slot_bytes * words;
+      long by_lds = lds > 0 ? FUSED_LDS_BYTES / lds : 0;
+      return std::min<long>(by_lds, 32 / std::max(1, T / 64));
+    };
+    products = resident(ndk) >= resident(1);
+  }
 
   P.n = n;
   P.nlevel = nlevel;
@@ -406,8 +431,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.maxslots = maxslots;
   P.dict = use_dict;
   P.dictv = dict;
-  P.ndict = !use_dict ? 0 : dict.size() <= 1 ? 1 : dict.size() == 2 ? 2 : 4;
-  const uint32_t np = P.ndict ? (uint32_t)P.ndict : 1u;  // LDS words per slot
+  P.ndict = !use_dict ? 0 : ndk == 1 ? 1 : products ? ndk : -ndk;
+  const uint32_t np = P.ndict > 0 ? (uint32_t)P.ndict : 1u;  // LDS words per slot
   const uint32_t zero_slot = (uint32_t)maxslots * np;  // LDS index of the zero record (code 0)
   P.hdr.assign((size_t)npart * FUSED_HDR, 0);
   P.grow.clear();
@@ -444,7 +469,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
         } else {
           vseg[(size_t)r * W + k] = v;
         }
-        uint32_t e16 = (uint32_t)sl * np + code;  // LDS index of the value or of the product d_code * x
+        // product records: LDS index of d_code * x; select: slot | code << 14
+        uint32_t e16 = P.ndict < 0 ? (uint32_t)sl | (code << 14) : (uint32_t)sl * np + code;
         uint32_t &word = eseg[(size_t)r * W2 + k / 2];
         word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
       }

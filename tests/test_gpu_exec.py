@@ -75,7 +75,7 @@ def test_fused_dict_bitwise_oracle(s):
     x = M.uniform_pm1(0x5EED, n)
     gold = O.spmv_k(rp, ci, va, x, k)
     F = M.FusedMPK(rp, ci, va, part, k, s)
-    assert F.stats()["codec"] == "dict"
+    assert F.stats()["codec"] == "dict" and F.stats()["dict_codec"] == "products"
     for graph in (False, True):
         vv = zeros_dev(n * (k + 1))
         F.exec(to_dev(x), vv, use_graph=graph)
@@ -151,6 +151,7 @@ def test_headline_size_128_cube_bitwise_oracle():
     assert np.array_equal(to_host(vv), gold)
     part = M.partition_grid([128, 128, 128], [8, 8, 8])
     F = M.FusedMPK(rp, ci, va, part, k, 3)
+    assert F.stats()["dict_codec"] == "select"  # 16^3 tiles: product records would not fit the LDS
     vv2 = zeros_dev(n * (k + 1))
     F.exec(to_dev(x), vv2)
     out = to_host(vv2)
