@@ -75,7 +75,11 @@ def test_fused_dict_bitwise_oracle(s):
     x = M.uniform_pm1(0x5EED, n)
     gold = O.spmv_k(rp, ci, va, x, k)
     F = M.FusedMPK(rp, ci, va, part, k, s)
-    assert F.stats()["codec"] == "dict" and F.stats()["dict_codec"] == "products"
+    st = F.stats()
+    assert st["codec"] == "dict"
+    # 8^3 tiles: at s = 1 the product records keep 4 workgroups per CU resident,
+    # beyond that they would cost residency and the select codec is used
+    assert st["dict_codec"] == ("products" if s == 1 else "select"), st
     for graph in (False, True):
         vv = zeros_dev(n * (k + 1))
         F.exec(to_dev(x), vv, use_graph=graph)
