@@ -28,7 +28,6 @@ namespace {
 
 constexpr int SS_MAX_S = 8;
 constexpr int SS_MAXM = 2 * SS_MAX_S + 1;
-constexpr int SS_PAIRS = SS_MAXM * (SS_MAXM + 1) / 2;
 constexpr int GT = 256, GR = 64;  // threads, rows per Gram tile
 constexpr int GRID = 1024;
 
@@ -67,12 +66,21 @@ __global__ __launch_bounds__(GT) void k_gram_partial(const double *__restrict__ 
   if (t < npairs) partial[(size_t)blockIdx.x * npairs + t] = acc;
 }
 
-__global__ void k_gram_finish(const double *__restrict__ partial, int blocks, int npairs, double *__restrict__ g) {
-  const int t = threadIdx.x;
-  if (t >= npairs) return;
+// One workgroup per pair: lane t sums the partials t, t + 256, ..., then a
+// pairwise tree in LDS (a fixed order, so the Gram matrix is deterministic).
+__global__ __launch_bounds__(GT) void k_gram_finish(const double *__restrict__ partial, int blocks, int npairs,
+                                                    double *__restrict__ g) {
+  __shared__ double red[GT];
+  const int q = blockIdx.x, t = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < blocks; b++) s += partial[(size_t)b * npairs + t];
-  g[t] = s;
+  for (int b = t; b < blocks; b += GT) s += partial[(size_t)b * npairs + q];
+  red[t] = s;
+  __syncthreads();
+  for (int w = GT / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) g[q] = red[0];
 }
 
 // x += Y c[0], r = Y c[1], p = Y c[2]
@@ -177,7 +185,7 @@ int mpkx_sstep_solve(mpkx_sstep *S, const double *b, double *x, double rtol, int
     if ((rc = mpkx_fused_exec(S->mpk, S->p.p, S->vp.p, st, 0)) || (rc = mpkx_fused_exec(S->mpk, S->r.p, S->vr.p, st, 0)))
       return rc;
     hipLaunchKernelGGL(k_gram_partial, dim3(GRID), dim3(GT), 0, st, S->vp.p, np, S->vr.p, nr, n, S->partial.p);
-    hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(SS_PAIRS), 0, st, S->partial.p, GRID, npairs, S->gram.p);
+    hipLaunchKernelGGL(k_gram_finish, dim3(npairs), dim3(GT), 0, st, S->partial.p, GRID, npairs, S->gram.p);
     MPKX_HIP(hipGetLastError());
     MPKX_HIP(hipMemcpyAsync(gp.data(), S->gram.p, sizeof(double) * npairs, hipMemcpyDeviceToHost, st));
     MPKX_HIP(hipStreamSynchronize(st));
