@@ -445,6 +445,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_fused (one round of s levels)",
+                     "algorithmic_bytes_per_launch": w_dom * B,
+                     "traffic_gbs": None if traffic is None else round(traffic / (round_ms[r_dom] * 1e-3) / 1e9, 2),
+                     "traffic_frac": None if traffic is None else
+                     round(traffic / (round_ms[r_dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "round_ms": [round(t, 5) for t in round_ms]},
     }
     host_vv = vf.cpu().numpy()
