@@ -254,6 +254,8 @@ typedef struct mpkx_fused mpkx_fused;
 enum {
   MPKX_FUSED_AUTO = 0,     /* pick the value codec automatically          */
   MPKX_FUSED_EXPLICIT = 1, /* force explicit FP64 values                  */
+  MPKX_FUSED_PRODUCTS = 2, /* dictionary: product records in LDS           */
+  MPKX_FUSED_SELECT = 4,   /* dictionary: per-entry select                */
 };
 /* part[n] labels in [0, npart); s = levels per launch (1..8). */
 int mpkx_fused_create(mpkx_fused **f, int n, const int *ptr, const int *col,
@@ -321,7 +323,10 @@ int mpkx_pd_plan_stats(const mpkx_pd_plan *plan, int *nbatch, long *rows,
 /*   vv[l*n + i] = (A^l x)_i / sigma^l.                                 */
 /* ------------------------------------------------------------------ */
 typedef struct mpkx_fused32 mpkx_fused32;
-enum { MPKX_VALUES_AUTO = 0, MPKX_VALUES_FP32 = 1, MPKX_VALUES_BF16 = 2 };
+/* AUTO: dictionary when the matrix has <= 4 distinct values (codec picked */
+/* by LDS residency), else FP32; *_PRODUCTS / *_SELECT force the codec.    */
+enum { MPKX_VALUES_AUTO = 0, MPKX_VALUES_FP32 = 1, MPKX_VALUES_BF16 = 2,
+       MPKX_VALUES_AUTO_PRODUCTS = 3, MPKX_VALUES_AUTO_SELECT = 4 };
 int mpkx_fused32_create(mpkx_fused32 **f, int n, const int *ptr, const int *col,
                         const double *val, const int *part, int npart,
                         int nlevel, int s, int values, double *sigma);
@@ -330,7 +335,8 @@ void mpkx_fused32_destroy(mpkx_fused32 *f);
 int mpkx_fused32_exec(mpkx_fused32 *f, const double *x, float *vv,
                       void *stream, int use_graph);
 int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream);
-/* values: 0 dictionary, 1 FP32 stream, 2 BF16 stream. */
+/* values: 0 dictionary (product records), 1 FP32 stream, 2 BF16 stream,
+ * 3 dictionary (per-entry select). */
 int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots,
                        long *row_levels, long *matrix_bytes, int *values,
                        int *launches);

@@ -591,7 +591,7 @@ class CsrMatrix:
 class FusedMPK:
     """Fused skirt-round MPK: s powers per launch, tiles from `part`."""
 
-    EXPLICIT = 1
+    EXPLICIT, PRODUCTS, SELECT = 1, 2, 4
 
     def __init__(self, ptr, col, val, part, nlevel, s, flags=0):
         self._h = C.c_void_p()
@@ -647,7 +647,7 @@ class FusedMPK32:
     a dictionary (AUTO), FP32 or BF16; vv[l] = A^l x / sigma^l (exact power-of-two
     scaling per power)."""
 
-    AUTO, FP32, BF16 = 0, 1, 2
+    AUTO, FP32, BF16, AUTO_PRODUCTS, AUTO_SELECT = 0, 1, 2, 3, 4
 
     def __init__(self, ptr, col, val, part, nlevel, s, values=0):
         self._h = C.c_void_p()
@@ -674,7 +674,8 @@ class FusedMPK32:
         _check(lib().mpkx_fused32_stats(self._h, C.byref(ms), C.byref(sl), C.byref(rl), C.byref(mb),
                                         C.byref(vals), C.byref(la)))
         return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
-                "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16"][vals.value],
+                "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16", "dict"][vals.value],
+                "dict_codec": {0: "products", 3: "select"}.get(vals.value),
                 "launches": la.value, "sigma": self.sigma}
 
     def __del__(self, _release=_release):

@@ -63,7 +63,8 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   int rc;
   try {
     rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, !(flags & MPKX_FUSED_EXPLICIT),
-                         2 * (int)sizeof(double), {{512, 4}, {1024, 4}, {1024, 8}}, f->plan);
+                         2 * (int)sizeof(double), {{512, 4}, {1024, 4}, {1024, 8}}, f->plan,
+                         (flags & MPKX_FUSED_PRODUCTS) ? 1 : (flags & MPKX_FUSED_SELECT) ? 2 : 0);
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fused_create: host allocation");
   }

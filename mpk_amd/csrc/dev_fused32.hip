@@ -83,14 +83,17 @@ extern "C" {
 int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *col, const double *val,
                         const int *part, int npart, int nlevel, int s, int values, double *sigma_out) {
   clear_error();
-  if (!out || values < 0 || values > 2) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
+  if (!out || values < 0 || values > 4) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
+  const bool dict_ok = values == MPKX_VALUES_AUTO || values == MPKX_VALUES_AUTO_PRODUCTS ||
+                       values == MPKX_VALUES_AUTO_SELECT;
+  const int dict_codec = values == MPKX_VALUES_AUTO_PRODUCTS ? 1 : values == MPKX_VALUES_AUTO_SELECT ? 2 : 0;
   auto *f = new (std::nothrow) mpkx_fused32;
   if (!f) return fail(MPKX_ENOMEM, "fused32_create");
   std::unique_ptr<mpkx_fused32> guard(f);
   int rc;
   try {
-    rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, values == MPKX_VALUES_AUTO,
-                         2 * (int)sizeof(float), {{512, 4}, {1024, 4}, {1024, 8}, {1024, 16}}, f->plan);
+    rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, dict_ok, 2 * (int)sizeof(float),
+                         {{512, 4}, {1024, 4}, {1024, 8}, {1024, 16}}, f->plan, dict_codec);
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fused32_create: host allocation");
   }
@@ -180,7 +183,7 @@ int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots, long
   if (slots) *slots = f->slots;
   if (row_levels) *row_levels = f->plan.row_levels;
   if (matrix_bytes) *matrix_bytes = f->matrix_bytes;
-  if (values) *values = f->values;
+  if (values) *values = f->values == 0 && f->plan.ndict < 0 ? 3 : f->values;
   if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
   return MPKX_OK;
 }

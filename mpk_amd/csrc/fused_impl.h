@@ -26,11 +26,12 @@
 //           consecutive [row][word] (one wide load per row); row r = m*T +
 //           thread lives in thread r % T's registers for the whole launch.
 //           With a dictionary of ND <= 4 distinct values (kernel arguments)
-//           there are two codecs.  Product records: the LDS holds, for every
-//           slot, the ND products d_c * x (each rounded as the reference
-//           rounds a_ij * x_j), and an entry is the LDS index slot * ND + c
-//           of its product: a row is W loads and W adds, the products are
-//           formed once per slot by its writer.  Select: the LDS holds x,
+//           there are two codecs.  Product records: the LDS holds ND planes
+//           of maxslots+1 words, plane c holding d_c * x for every slot (each
+//           rounded as the reference rounds a_ij * x_j), and an entry is the
+//           LDS index c * (maxslots+1) + slot of its product: a row is W
+//           loads and W adds, the products are formed once per slot by its
+//           writer.  Select: the LDS holds x,
 //           an entry is the slot plus a 2-bit code in bits 14-15, and the
 //           value is selected from the dictionary per entry (less LDS, more
 //           VALU; chosen when product records would cost occupancy).
@@ -62,9 +63,9 @@ constexpr int FUSED_HDR = 16;       // ints per tile header
 // header: [0] row-list offset, [1] ELL offset, [2] value offset, [3] region
 // rows, [4 .. 4+s] cnt, [15] row base (added to every stored row number)
 constexpr int FUSED_MAX_S = 10;     // levels per launch (header holds cnt[0..s])
-// region rows per tile: an entry is a 16-bit LDS index slot * ND + c
-// (ND <= 4), and slot maxslots is the zero record: it holds 0 (or its
-// products) in both LDS buffers and padding entries point to it
+// region rows per tile: an entry is a 16-bit LDS index c * (maxslots+1) +
+// slot (ND <= 4 planes), and slot maxslots is the zero record: it holds 0
+// (or its products) in both LDS buffers and padding entries point to it
 constexpr int FUSED_MAX_SLOTS = 16382;
 constexpr long FUSED_LDS_BYTES = 160 * 1024;
 static_assert(FUSED_HDR >= 4 + FUSED_MAX_S + 2, "header holds cnt[0..s] and the row base");
@@ -87,20 +88,18 @@ __device__ __forceinline__ float mul(float a, float x) { return a * x; }
 __device__ __forceinline__ double add(double s, double p) { return __dadd_rn(s, p); }
 __device__ __forceinline__ float add(float s, float p) { return s + p; }
 
-// LDS record of one slot: the value (ND <= 0) or its ND dictionary products.
+// LDS record of one slot: the value (ND <= 0) or its ND dictionary
+// products, one per plane of `plane` words.
 template <typename VT, int ND>
-__device__ __forceinline__ void put_slot(VT *buf, int slot, VT v, VT d0, VT d1, VT d2, VT d3) {
+__device__ __forceinline__ void put_slot(VT *buf, int slot, int plane, VT v, VT d0, VT d1, VT d2, VT d3) {
   if constexpr (ND <= 0) {
     buf[slot] = v;
-  } else if constexpr (ND == 1) {
-    buf[slot] = mul(d0, v);
   } else {
-    VT *q = buf + (size_t)slot * ND;
-    q[0] = mul(d0, v);
-    q[1] = mul(d1, v);
+    buf[slot] = mul(d0, v);
+    if constexpr (ND >= 2) buf[plane + slot] = mul(d1, v);
     if constexpr (ND == 4) {
-      q[2] = mul(d2, v);
-      q[3] = mul(d3, v);
+      buf[2 * plane + slot] = mul(d2, v);
+      buf[3 * plane + slot] = mul(d3, v);
     }
   }
 }
@@ -143,11 +142,11 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
 #pragma unroll
     for (int u = 0; u < GB; u++)
-      if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, gv[u], d0, d1, d2, d3);
+      if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, maxslots + 1, gv[u], d0, d1, d2, d3);
   }
   if (tid == 0) {
-    put_slot<VT, ND>(buf0, maxslots, (VT)0, d0, d1, d2, d3);
-    put_slot<VT, ND>(buf1, maxslots, (VT)0, d0, d1, d2, d3);
+    put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
+    put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
   }
 
   // matrix rows of this thread into registers (W2 consecutive words per
@@ -222,7 +221,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
           }
         }
         if (SCALE) s = s * inv_sigma;
-        put_slot<VT, ND>(out, r, s, d0, d1, d2, d3);
+        put_slot<VT, ND>(out, r, maxslots + 1, s, d0, d1, d2, d3);
         if (own[m] >= 0) dst[own[m]] = s;
       }
       // keep the next row's LDS loads below this point: one row's operands
@@ -301,9 +300,11 @@ long share_segment(std::vector<V> &store, std::unordered_multimap<uint64_t, long
 
 // Builds the plan.  slot_bytes: LDS bytes per region row (2 buffers).
 // mr_choices: allowed (T, MR) pairs, smallest first.
+// dict_codec: 0 automatic, 1 product records, 2 select (dictionary plans).
 inline int build_tile_plan(int n, const int *ptr, const int *col, const double *val, const int *part,
                            int npart, int nlevel, int s, bool allow_dict, int slot_bytes,
-                           const std::vector<std::pair<int, int>> &mr_choices, TilePlan &P) {
+                           const std::vector<std::pair<int, int>> &mr_choices, TilePlan &P,
+                           int dict_codec = 0) {
   if (n <= 0 || !ptr || !col || !val || !part || npart <= 0 || nlevel <= 0)
     return fail(MPKX_EINVAL, "fused: bad arguments");
   if (s < 1 || s > FUSED_MAX_S) return fail(MPKX_EINVAL, "fused: s=%d outside 1..%d", s, FUSED_MAX_S);
@@ -418,7 +419,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
       long by_lds = lds > 0 ? FUSED_LDS_BYTES / lds : 0;
       return std::min<long>(by_lds, 32 / std::max(1, T / 64));
     };
-    products = resident(ndk) >= resident(1);
+    products = dict_codec == 1 ? resident(ndk) > 0 : dict_codec == 2 ? false : resident(ndk) >= resident(1);
   }
 
   P.n = n;
@@ -432,8 +433,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.dict = use_dict;
   P.dictv = dict;
   P.ndict = !use_dict ? 0 : ndk == 1 ? 1 : products ? ndk : -ndk;
-  const uint32_t np = P.ndict > 0 ? (uint32_t)P.ndict : 1u;  // LDS words per slot
-  const uint32_t zero_slot = (uint32_t)maxslots * np;  // LDS index of the zero record (code 0)
+  const uint32_t plane = (uint32_t)maxslots + 1;  // LDS words per product plane
+  const uint32_t zero_slot = (uint32_t)maxslots;   // the zero record (plane 0)
   P.hdr.assign((size_t)npart * FUSED_HDR, 0);
   P.grow.clear();
   P.ent.clear();
@@ -470,7 +471,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
           vseg[(size_t)r * W + k] = v;
         }
         // product records: LDS index of d_code * x; select: slot | code << 14
-        uint32_t e16 = P.ndict < 0 ? (uint32_t)sl | (code << 14) : (uint32_t)sl * np + code;
+        uint32_t e16 = P.ndict < 0 ? (uint32_t)sl | (code << 14) : code * plane + (uint32_t)sl;
         uint32_t &word = eseg[(size_t)r * W2 + k / 2];
         word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
       }

@@ -45,8 +45,13 @@ def time_fused(F, xd, vv, reps, warmup=2):
     return e0.elapsed_time(e1) / reps
 
 
-def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, reps=10, x=None, log=None):
-    """Times every (blocks, s) candidate; returns a list of dicts sorted by ms."""
+CODECS = {"auto": (0, 0), "products": (2, 3), "select": (4, 4)}  # name -> (FP64 flags, FP32 values)
+
+
+def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, reps=10, x=None, log=None,
+              codecs=("auto",)):
+    """Times every (blocks, s, dictionary codec) candidate; returns a list of
+    dicts sorted by ms."""
     import torch
     n = len(ptr) - 1
     if x is None:
@@ -55,15 +60,17 @@ def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, rep
     dt = torch.float64 if precision == "f64" else torch.float32
     vv = torch.zeros(n * (nlevel + 1), dtype=dt, device="cuda:0")
     res = []
-    for blocks, s in (candidates or grid_candidates(dims)):
-        rec = {"blocks": list(blocks), "s": s, "precision": precision}
+    for (blocks, s), codec in itertools.product(candidates or grid_candidates(dims), codecs):
+        rec = {"blocks": list(blocks), "s": s, "precision": precision, "codec": codec}
         t0 = time.perf_counter()
         try:
             part = M.partition_grid(dims, blocks)
-            F = (M.FusedMPK(ptr, col, val, part, nlevel, s) if precision == "f64"
-                 else M.FusedMPK32(ptr, col, val, part, nlevel, s))
+            f64, f32 = CODECS[codec]
+            F = (M.FusedMPK(ptr, col, val, part, nlevel, s, f64) if precision == "f64"
+                 else M.FusedMPK32(ptr, col, val, part, nlevel, s, f32))
             rec["ms"] = time_fused(F, xd, vv, reps)
-            rec.update({k: v for k, v in F.stats().items() if k in ("max_slots", "row_levels", "matrix_bytes")})
+            rec.update({k: v for k, v in F.stats().items()
+                        if k in ("max_slots", "row_levels", "matrix_bytes", "dict_codec")})
             del F
         except M.MpkxError as e:
             rec["unsupported"] = str(e)

@@ -127,6 +127,24 @@ def test_fused_shared_segments_bitwise_oracle(blocks, s):
     assert np.array_equal(to_host(vv), gold)
 
 
+@pytest.mark.parametrize("flags", [M.FusedMPK.PRODUCTS, M.FusedMPK.SELECT])
+def test_fused_forced_codec_bitwise_oracle(flags):
+    """Both dictionary codecs give the reference's bits: product records are
+    the same roundings of a_ij * x_j, formed once per slot."""
+    say("fused forced codec", flags)
+    rp, ci, va = M.gen2("cube", 40, -1.0)
+    n = len(rp) - 1
+    part = M.partition_grid([40, 40, 40], [4, 4, 4])
+    k = 7
+    x = M.uniform_pm1(9, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    F = M.FusedMPK(rp, ci, va, part, k, 3, flags)
+    assert F.stats()["dict_codec"] == {M.FusedMPK.PRODUCTS: "products", M.FusedMPK.SELECT: "select"}[flags]
+    vv = zeros_dev(n * (k + 1))
+    F.exec(to_dev(x), vv)
+    assert np.array_equal(to_host(vv), gold)
+
+
 def test_fused_graph_partition_bitwise_oracle():
     say("fused graph partition")
     rp, ci, va = M.gen2("cube", 24, -1.0)
@@ -189,7 +207,7 @@ def _rel_errs_scaled(out32, gold, n, k, sigma):
     return np.array(errs)
 
 
-@pytest.mark.parametrize("values,tol", [("auto", 1e-5), ("fp32", 1e-5)])
+@pytest.mark.parametrize("values,tol", [("auto", 1e-5), ("fp32", 1e-5), ("products", 1e-5), ("select", 1e-5)])
 def test_fused32_cube_per_power_error(values, tol):
     """FP32 form on the 7-pt Laplacian (values exact in FP32): error vs the FP64
     oracle stays at FP32 rounding level for every power."""
@@ -201,7 +219,9 @@ def test_fused32_cube_per_power_error(values, tol):
     x = M.uniform_pm1(0x5EED, n)
     gold = O.spmv_k(rp, ci, va, x, k)
     part = M.partition_grid([N] * 3, [4, 4, 4])
-    F = M.FusedMPK32(rp, ci, va, part, k, 4, {"auto": 0, "fp32": 1}[values])
+    F = M.FusedMPK32(rp, ci, va, part, k, 4, {"auto": 0, "fp32": 1, "products": 3, "select": 4}[values])
+    if values in ("products", "select"):
+        assert F.stats()["dict_codec"] == values
     assert F.sigma == 16.0
     vv = torch.zeros(n * (k + 1), dtype=torch.float32, device="cuda")
     F.exec(to_dev(x), vv)
