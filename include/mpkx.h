@@ -324,7 +324,8 @@ int mpkx_pd_plan_stats(const mpkx_pd_plan *plan, int *nbatch, long *rows,
 /* ------------------------------------------------------------------ */
 typedef struct mpkx_fused32 mpkx_fused32;
 /* AUTO: dictionary when the matrix has <= 4 distinct values (codec picked */
-/* by LDS residency), else FP32; *_PRODUCTS / *_SELECT force the codec.    */
+/* by LDS residency), else FP32; *_PRODUCTS / *_SELECT force the codec   */
+/* (product records fall back to select when they do not fit the LDS).  */
 enum { MPKX_VALUES_AUTO = 0, MPKX_VALUES_FP32 = 1, MPKX_VALUES_BF16 = 2,
        MPKX_VALUES_AUTO_PRODUCTS = 3, MPKX_VALUES_AUTO_SELECT = 4 };
 int mpkx_fused32_create(mpkx_fused32 **f, int n, const int *ptr, const int *col,

@@ -218,7 +218,9 @@ def test_fused32_cube_per_power_error(values, tol):
     n = len(rp) - 1
     x = M.uniform_pm1(0x5EED, n)
     gold = O.spmv_k(rp, ci, va, x, k)
-    part = M.partition_grid([N] * 3, [4, 4, 4])
+    # 16^3 tiles (s = 4) leave no LDS room for product records; the forced
+    # codecs run on 8^3 tiles
+    part = M.partition_grid([N] * 3, [8, 8, 8] if values in ("products", "select") else [4, 4, 4])
     F = M.FusedMPK32(rp, ci, va, part, k, 4, {"auto": 0, "fp32": 1, "products": 3, "select": 4}[values])
     if values in ("products", "select"):
         assert F.stats()["dict_codec"] == values
