@@ -237,6 +237,35 @@ def sstep_cg(M, torch, dev, rp, ci, va, N, blocks, rtol=1e-8):
     return out
 
 
+def pdmpk_form(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps):
+    """pdmpk's partial-diamond schedule (pdmpk/comm_comp_patterns.cc) on the
+    same workload: 64 partitions (4x4x4 blocks), one redrawn partition
+    (shifted blocks), final power only; one hipGraph of all batches."""
+    n = len(rp) - 1
+    part0 = M.partition_grid([N] * 3, [4, 4, 4])
+    reps = [M.partition_grid([N] * 3, [4, 4, 4], [N // 8] * 3)]
+    t0 = time.perf_counter()
+    pd = M.PdBuffers.prep(rp, ci, va, 64, k, part0, reps, 0)
+    t_prep = time.perf_counter() - t0
+    plan = M.PdPlan(pd)
+    xd = torch.from_numpy(x).to(dev)
+    y = torch.zeros(n, dtype=torch.float64, device=dev)
+    for _ in range(3):
+        plan.exec(xd, y, send_home=False)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan.exec(xd, y, send_home=False)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    st = plan.stats()
+    return {"workload": f"pdmpk schedule, {N}^3 7-pt, k={k}, 64 partitions, final power only",
+            "ms_per_step": round(ms, 5), "vs_kxspmv": round(ms_plain / ms, 3), "batches": st["batches"],
+            "rows": st["rows"], "nnz": st["nnz"], "ghost_copies": st["ghosts"], "prep_seconds": round(t_prep, 2)}
+
+
 def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
     """k back-to-back SpMVs through torch.sparse CSR (hipSPARSE/rocSPARSE on
     ROCm), as an outside reference point for the plain kernel."""
@@ -279,6 +308,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-matrix form")
     ap.add_argument("--no-sstep", action="store_true", help="skip the s-step CG consumer")
+    ap.add_argument("--no-pdmpk", action="store_true", help="skip the pdmpk schedule")
     ap.add_argument("--batch-config", default=None,
                     help="B,L,T for the batched form instead of the exhaustive autotuner (profiling runs)")
     ap.add_argument("--no-tuning", action="store_true", help="skip the learned-tuning evaluation")
@@ -454,6 +484,12 @@ def main():
     host_vv = vf.cpu().numpy()
     del F, A, vf, vp
     torch.cuda.empty_cache()
+    if not args.no_pdmpk:
+        try:
+            out["pdmpk_form"] = pdmpk_form(M, torch, dev, rp, ci, va, x, N, k, ms_plain, args.steps)
+        except Exception as e:
+            out["pdmpk_form"] = {"error": str(e)}
+        torch.cuda.empty_cache()
     if not args.no_sstep:
         try:
             out["sstep_cg"] = sstep_cg(M, torch, dev, rp, ci, va, N, blocks)

@@ -304,12 +304,14 @@ int mpkx_pd_load(mpkx_pd **pd, const char *prefix, int npart, int n,
 /* GPU exec (Buffers::Exec, pdmpk/buffers.cc:52-97): all partitions' mbufs
  * in one HBM array, one launch per batch, DoComm as a ghost copy.
  * LoadInput takes x (device, n); y (device, n) receives the final power
- * (Results::FillResults).  send_home != 0 also runs SendHome. */
+ * (Results::FillResults).  send_home != 0 also runs SendHome.  use_graph
+ * != 0 records the launches once per (x, y, stream, send_home) into a
+ * hipGraph and replays it. */
 typedef struct mpkx_pd_plan mpkx_pd_plan;
 int mpkx_pd_plan_create(mpkx_pd_plan **plan, const mpkx_pd *pd);
 void mpkx_pd_plan_destroy(mpkx_pd_plan *plan);
 int mpkx_pd_exec(mpkx_pd_plan *plan, const double *x, double *y,
-                 int send_home, void *stream);
+                 int send_home, void *stream, int use_graph);
 int mpkx_pd_plan_stats(const mpkx_pd_plan *plan, int *nbatch, long *rows,
                        long *nnz, long *ghosts);
 

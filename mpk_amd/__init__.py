@@ -103,7 +103,7 @@ def lib():
         "mpkx_pd_load": ([P, C.c_char_p, C.c_int, C.c_int, C.c_int], C.c_int),
         "mpkx_pd_plan_create": ([P, P], C.c_int),
         "mpkx_pd_plan_destroy": ([P], None),
-        "mpkx_pd_exec": ([P, P, P, C.c_int, P], C.c_int),
+        "mpkx_pd_exec": ([P, P, P, C.c_int, P, C.c_int], C.c_int),
         "mpkx_pd_plan_stats": ([P, P, P, P, P], C.c_int),
         "mpkx_tasks_create": ([P, C.c_int, P, P, P, C.c_int, C.c_int, C.c_int, P, P], C.c_int),
         "mpkx_tasks_destroy": ([P], None),
@@ -535,8 +535,8 @@ class PdPlan:
         self._h = C.c_void_p()
         _check(lib().mpkx_pd_plan_create(C.byref(self._h), pdbufs._h))
 
-    def exec(self, x, y, send_home=True, stream=None):
-        _check(lib().mpkx_pd_exec(self._h, _dev(x), _dev(y), int(send_home), _stream(stream)))
+    def exec(self, x, y, send_home=True, stream=None, use_graph=True):
+        _check(lib().mpkx_pd_exec(self._h, _dev(x), _dev(y), int(send_home), _stream(stream), int(use_graph)))
 
     def stats(self):
         nb, r, z, g = C.c_int(), C.c_long(), C.c_long(), C.c_long()

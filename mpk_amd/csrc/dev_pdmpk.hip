@@ -82,7 +82,10 @@ struct mpkx_pd_plan {
   DevArray<long> rp;
   DevArray<int> ci, seed, out_row, blk, cdst, csrc, xdst, xsrc, shdst, shsrc, rdst, rsrc;
   DevArray<double> va, mb, stage;
+  GraphCache graph;
 };
+
+static void pd_launches(mpkx_pd_plan *pl, const double *x, double *y, int send_home, hipStream_t st);
 
 extern "C" {
 
@@ -214,10 +217,40 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
 
 void mpkx_pd_plan_destroy(mpkx_pd_plan *pl) { delete pl; }
 
-int mpkx_pd_exec(mpkx_pd_plan *pl, const double *x, double *y, int send_home, void *stream) {
+int mpkx_pd_exec(mpkx_pd_plan *pl, const double *x, double *y, int send_home, void *stream, int use_graph) {
   clear_error();
   if (!pl || !x || !y) return fail(MPKX_EINVAL, "pd_exec: bad arguments");
   hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    pd_launches(pl, x, y, send_home, st);
+    MPKX_HIP(hipGetLastError());
+    return MPKX_OK;
+  }
+  if (!pl->graph.matches(x, y, st, send_home != 0)) {
+    pl->graph.reset();
+    hipStream_t cap = st;
+    bool own = false;
+    if (cap == nullptr) {  // the legacy default stream cannot be captured
+      MPKX_HIP(hipStreamCreate(&cap));
+      own = true;
+    }
+    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    pd_launches(pl, x, y, send_home, cap);
+    MPKX_HIP(hipStreamEndCapture(cap, &pl->graph.graph));
+    MPKX_HIP(hipGraphInstantiate(&pl->graph.exec, pl->graph.graph, nullptr, nullptr, 0));
+    if (own) MPKX_HIP(hipStreamDestroy(cap));
+    pl->graph.k0 = x;
+    pl->graph.k1 = y;
+    pl->graph.stream = st;
+    pl->graph.k = send_home != 0;
+  }
+  MPKX_HIP(hipGraphLaunch(pl->graph.exec, st));
+  return MPKX_OK;
+}
+
+}  // extern "C"
+
+static void pd_launches(mpkx_pd_plan *pl, const double *x, double *y, int send_home, hipStream_t st) {
   // LoadInput: mbuf[i] = x[original_idcs[i]] (the reference loads ones)
   if (pl->nhome > 0) {
     hipLaunchKernelGGL(k_pd_gather, dim3(g1(pl->nhome)), dim3(256), 0, st, pl->stage.p, x, pl->xsrc.p, pl->nhome);
@@ -248,9 +281,9 @@ int mpkx_pd_exec(mpkx_pd_plan *pl, const double *x, double *y, int send_home, vo
     hipLaunchKernelGGL(k_pd_scatter, dim3(g1(pl->nsh)), dim3(256), 0, st, pl->mb.p, pl->stage.p, pl->shdst.p,
                        pl->nsh);
   }
-  MPKX_HIP(hipGetLastError());
-  return MPKX_OK;
 }
+
+extern "C" {
 
 int mpkx_pd_plan_stats(const mpkx_pd_plan *pl, int *nbatch, long *rows, long *nnz, long *ghosts) {
   clear_error();
