@@ -32,6 +32,19 @@
  *   orc_vexec_tasks     bitwise equal to orc_spmv_k (shared-memory path has
  *                       no reference output file to compare with: the
  *                       reference mpkrun reads stale file names)
+ *
+ * Functions with no reference counterpart (checkers of this project's
+ * extensions), pinned by mathematics instead:
+ *   orc_pow2_sigma      hand-computed cases, exact powers of two kept
+ *   orc_batch_mpk32     integer data (every FP32 sum exact) must equal the
+ *                       FP64 powers of A/sigma for every lane split; a
+ *                       hand-computed row separates the summation orders;
+ *                       <= 1e-5 from FP64 on random power-law data
+ *   orc_mlp_loss_grad   central finite differences of the FP64 loss; the
+ *                       zero-weight closed form (y = b3)
+ *   orc_cg              the solution of a small SPD system by a direct
+ *                       sparse solve; the iteration count bound of CG in
+ *                       exact arithmetic (n) on a tiny system
  */
 #include <stdint.h>
 #include <stdio.h>

@@ -232,3 +232,23 @@ def test_suite_is_reproducible():
     assert {s["kind"] for s in a} == {"stencil", "banded", "chunglu"}
     rp, ci, _ = CM.make_matrix(a[1])
     assert len(rp) - 1 == a[1]["n"]
+
+
+def test_oracle_cg_against_direct_solve():
+    """The CG checker: matches a direct sparse solve on an SPD stencil, and on
+    a 4x4 grid (n = 16) converges within n iterations."""
+    import scipy.sparse.linalg as spla
+    rp, ci, va = O.gen2("cube", 8)
+    va = -va  # diag +6, off -1: SPD
+    n = len(rp) - 1
+    b = O.uniform(3, n)
+    x, it = O.cg(rp, ci, va, b, 1e-12)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-10
+    assert 1 < it < n
+    rp, ci, va = O.gen2("m5p", 4)
+    va = -va
+    b = O.uniform(4, 16)
+    x, it = O.cg(rp, ci, va, b, 1e-13)
+    assert it <= 16
