@@ -14,7 +14,6 @@
 // equal to the reference exec (and to the oracle restatement of it).
 #include <algorithm>
 #include <memory>
-#include <unordered_set>
 
 #include "dev_common.h"
 #include "pdmpk.h"
@@ -83,6 +82,7 @@ struct mpkx_pd_plan {
   DevArray<int> ci, seed, out_row, blk, cdst, csrc, xdst, xsrc, shdst, shsrc, rdst, rsrc;
   DevArray<double> va, mb, stage;
   GraphCache graph;
+  long aliased = 0;  // ghost values read in place instead of copied
 };
 
 static void pd_launches(mpkx_pd_plan *pl, const double *x, double *y, int send_home, hipStream_t st);
@@ -112,7 +112,16 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
   std::vector<long> rp(1, 0);
   std::vector<int> ci, seed, outr, blk, cdst, csrc, xdst, xsrc, shdst, shsrc, rdst, rsrc;
   std::vector<double> va;
+  // Every mbuf slot is written once per exec (LoadInput, one copy or one
+  // row), and the exec's only output is the final power, so a ghost slot
+  // filled by DoComm can be read where its value lives: alias[slot] is that
+  // place, and rows, seeds and results read through it.  DoComm then needs
+  // no copy at all; the sums read the same values, so results are unchanged.
+  std::vector<int> alias;
+  long aliased = 0;
   try {
+    alias.resize(pl->flat);
+    for (long i = 0; i < pl->flat; i++) alias[i] = (int)i;
     for (int b = 0; b < pl->nbatch; b++) {
       // DoComm(b): rbuf of batch b follows the batch's computed rows
       pl->cbegin.push_back((long)cdst.size());
@@ -132,13 +141,17 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
               csrc.push_back((int)(base[s] + si));
             }
           }
-        std::unordered_set<int> dset(cdst.begin() + pl->cbegin[b], cdst.end());
-        bool staged = false;
-        for (size_t q = pl->cbegin[b]; q < csrc.size() && !staged; q++) staged = dset.count(csrc[q]) > 0;
-        pl->cstaged.push_back(staged);
-      } else {
-        pl->cstaged.push_back(0);
+        // all sends of a batch are packed before any receive lands: resolve
+        // every source through the map as it was before this batch
+        std::vector<int> src_now;
+        src_now.reserve(cdst.size() - pl->cbegin[b]);
+        for (size_t q = pl->cbegin[b]; q < cdst.size(); q++) src_now.push_back(alias[csrc[q]]);
+        for (size_t q = pl->cbegin[b]; q < cdst.size(); q++) alias[cdst[q]] = src_now[q - pl->cbegin[b]];
+        aliased += (long)src_now.size();
+        cdst.resize(pl->cbegin[b]);
+        csrc.resize(pl->cbegin[b]);
       }
+      pl->cstaged.push_back(0);
       // DoComp(b) of every partition (buffers.cc:76-97)
       int r0 = (int)rp.size() - 1;
       for (int p = 0; p < np; p++) {
@@ -148,13 +161,15 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
         for (size_t mi = 0; mi < m1 - m0; mi++) {
           int sd = -1;
           if (ii < ie && (long)(q.mbuf_begin[b] + mi) == q.init_idcs[ii].second) {
-            sd = (int)(base[p] + q.init_idcs[ii].first);
+            if (q.init_idcs[ii].first < 0 || q.init_idcs[ii].first >= q.mbuf_idx)
+              return fail(MPKX_EINVAL, "pd_plan_create: init index out of range");
+            sd = alias[base[p] + q.init_idcs[ii].first];
             ii++;
           }
           int ja = q.mptr[m0 + mi], jb = q.mptr[m0 + mi + 1];
           for (int j = ja; j < jb; j++) {
             if (q.mcol[j] < 0 || q.mcol[j] >= q.mbuf_idx) return fail(MPKX_EINVAL, "pd_plan_create: mcol out of range");
-            ci.push_back((int)(base[p] + q.mcol[j]));
+            ci.push_back(alias[base[p] + q.mcol[j]]);
             va.push_back(q.mval[j]);
           }
           rp.push_back((long)ci.size());
@@ -182,16 +197,22 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
       long nhome = (long)q.mbuf_begin[0];  // LoadInput: the home vertices' level 0
       if (nhome > (long)q.original_idcs.size()) return fail(MPKX_EINVAL, "pd_plan_create: original_idcs short");
       for (long i = 0; i < nhome; i++) {
+        if (q.original_idcs[i] < 0 || q.original_idcs[i] >= pl->n)
+          return fail(MPKX_EINVAL, "pd_plan_create: original index out of range");
         xdst.push_back((int)(base[p] + i));
         xsrc.push_back(q.original_idcs[i]);
       }
       for (size_t i = 0; i < q.home_idcs.size(); i++) {  // SendHome
+        if (q.home_idcs[i] < 0 || q.home_idcs[i] >= q.mbuf_idx) return fail(MPKX_EINVAL, "pd_plan_create: home index");
         shdst.push_back((int)(base[p] + i));
-        shsrc.push_back((int)(base[p] + q.home_idcs[i]));
+        shsrc.push_back(alias[base[p] + q.home_idcs[i]]);
       }
       for (size_t i = 0; i < q.res_mbuf_idcs.size(); i++) {  // Results::FillVal / FillResults
+        if (q.res_mbuf_idcs[i] < 0 || q.res_mbuf_idcs[i] >= q.mbuf_idx || i >= q.res_original_idcs.size() ||
+            q.res_original_idcs[i] < 0 || q.res_original_idcs[i] >= pl->n)
+          return fail(MPKX_EINVAL, "pd_plan_create: result index");
         rdst.push_back(q.res_original_idcs[i]);
-        rsrc.push_back((int)(base[p] + q.res_mbuf_idcs[i]));
+        rsrc.push_back(alias[base[p] + q.res_mbuf_idcs[i]]);
       }
     }
   } catch (const std::bad_alloc &) {
@@ -202,7 +223,8 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
   pl->nres = (long)rdst.size();
   long stage_n = 1;
   for (size_t b = 0; b + 1 < pl->cbegin.size(); b++) stage_n = std::max(stage_n, pl->cbegin[b + 1] - pl->cbegin[b]);
-  stage_n = std::max(stage_n, pl->nsh);
+  stage_n = std::max({stage_n, pl->nsh, pl->nhome, pl->nres});
+  pl->aliased = aliased;
   int rc;
   if ((rc = pl->rp.upload(rp)) || (rc = pl->ci.upload(ci)) || (rc = pl->va.upload(va)) ||
       (rc = pl->seed.upload(seed)) || (rc = pl->out_row.upload(outr)) || (rc = pl->blk.upload(blk)) ||
@@ -291,7 +313,7 @@ int mpkx_pd_plan_stats(const mpkx_pd_plan *pl, int *nbatch, long *rows, long *nn
   if (nbatch) *nbatch = pl->nbatch;
   if (rows) *rows = (long)pl->out_row.count;
   if (nnz) *nnz = (long)pl->ci.count;
-  if (ghosts) *ghosts = (long)pl->cdst.count;
+  if (ghosts) *ghosts = pl->aliased;
   return MPKX_OK;
 }
 
