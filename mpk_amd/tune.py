@@ -17,13 +17,15 @@ import numpy as np
 import mpk_amd as M
 
 
-def grid_candidates(dims, tile_sides=(8, 12, 16, 24, 32), s_values=(1, 2, 3, 4, 5, 6)):
-    """(blocks per dim, s) pairs for tiles with sides from tile_sides."""
+def grid_candidates(dims, tile_sides=(8, 12, 16, 24, 32), s_values=(1, 2, 3, 4, 5, 6), all_orientations=False):
+    """(blocks per dim, s) pairs for tiles with sides from tile_sides (the
+    last dimension is the fastest-varying row index; by default one
+    orientation per shape, longest side slowest)."""
     out = []
     for sides in itertools.product(tile_sides, repeat=len(dims)):
         if any(d % t for d, t in zip(dims, sides)):
             continue
-        if list(sides) != sorted(sides, reverse=True):  # one orientation per shape
+        if not all_orientations and list(sides) != sorted(sides, reverse=True):
             continue
         blocks = [d // t for d, t in zip(dims, sides)]
         for s in s_values:
