@@ -297,10 +297,10 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--s", type=int, default=2, help="levels per fused launch")
-    ap.add_argument("--blocks", default="8,16,16",
+    ap.add_argument("--blocks", default="16,16,8",
                     help="tiles per dimension (slowest first); default from the autotuner sweep")
     ap.add_argument("--s32", type=int, default=2, help="levels per launch, FP32 form")
-    ap.add_argument("--blocks32", default="8,16,16", help="tiles per dimension, FP32 form")
+    ap.add_argument("--blocks32", default="16,16,4", help="tiles per dimension, FP32 form")
     ap.add_argument("--calib", action="store_true",
                     help="also run one 512 MiB k_dcopy (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cold-reps", type=int, default=10, help="cold-MALL repetitions (0: skip)")
