@@ -266,6 +266,63 @@ def pdmpk_form(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps):
             "rows": st["rows"], "nnz": st["nnz"], "ghost_copies": st["ghosts"], "prep_seconds": round(t_prep, 2)}
 
 
+def precision_form(M, torch, dev, N=2048, k=16, blocks=(32, 32), s=2, steps=10):
+    """The precision workload: 2D 9-point stencil 2048^2 (4,194,304 rows),
+    off-diagonals -(1 + 0.1 N(0,1)) (numpy default_rng seed 5), diagonal = row
+    abs-sum + 0.5, k = 16, x from SplitMix64 seed 0x5EED.  FP64 fused (bitwise
+    equal to k x SpMV) is the reference; the FP32 and BF16 value streams
+    accumulate in FP32 with exact power-of-two renormalisation, and their
+    error is stated per power against the FP64 block (both on the GPU)."""
+    rp, ci, va = M.gen2("m9p", N, -1.0)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = ci != rows
+    rng = np.random.default_rng(5)
+    va = va.copy()
+    va[off] = -(1.0 + 0.1 * rng.standard_normal(int(off.sum())))
+    va[~off] = np.bincount(rows, weights=np.abs(va) * off, minlength=n) + 0.5
+    x = M.uniform_pm1(0x5EED, n)
+    xd = torch.from_numpy(x).to(dev)
+    part = M.partition_grid([N, N], list(blocks))
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    A = M.CsrMatrix(rp, ci, va)
+    vp = torch.zeros(n * (k + 1), dtype=torch.float64, device=dev)
+    vp[:n] = xd
+    ms_plain = timed(lambda: A.spmv_k(vp, k))
+    F = M.FusedMPK(rp, ci, va, part, k, s)
+    v64 = torch.zeros_like(vp)
+    ms64 = timed(lambda: F.exec(xd, v64))
+    out = {"workload": f"2D 9-pt {N}^2 (n={n}, nnz={len(ci)}), random off-diagonals, k={k}",
+           "kxspmv_fp64_ms": round(ms_plain, 4),
+           "fused_fp64": {"ms": round(ms64, 4), "speedup_vs_kxspmv": round(ms_plain / ms64, 3),
+                          "bitwise_equal_kxspmv": bool(torch.equal(v64, vp)), "codec": F.stats()["codec"]}}
+    del F, A, vp
+    ref = v64.view(k + 1, n)
+    for name, code in (("fp32", M.FusedMPK32.FP32), ("bf16", M.FusedMPK32.BF16)):
+        F32 = M.FusedMPK32(rp, ci, va, part, k, s, code)
+        v32 = torch.zeros(n * (k + 1), dtype=torch.float32, device=dev)
+        ms = timed(lambda: F32.exec(xd, v32))
+        scale = torch.tensor([F32.sigma ** l for l in range(k + 1)], dtype=torch.float64, device=dev)
+        errs = (torch.linalg.norm(v32.view(k + 1, n).double() * scale[:, None] - ref, dim=1)
+                / torch.linalg.norm(ref, dim=1)).cpu().numpy()
+        out[f"fused_{name}"] = {"ms": round(ms, 4), "speedup_vs_kxspmv_fp64": round(ms_plain / ms, 3),
+                               "sigma": F32.sigma, "per_power_rel_err_vs_fp64": [float("%.2e" % e) for e in errs]}
+        del F32, v32
+    return out
+
+
 def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
     """k back-to-back SpMVs through torch.sparse CSR (hipSPARSE/rocSPARSE on
     ROCm), as an outside reference point for the plain kernel."""
@@ -309,6 +366,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-matrix form")
     ap.add_argument("--no-sstep", action="store_true", help="skip the s-step CG consumer")
     ap.add_argument("--no-pdmpk", action="store_true", help="skip the pdmpk schedule")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 9-point FP64/FP32/BF16 form")
     ap.add_argument("--batch-config", default=None,
                     help="B,L,T for the batched form instead of the exhaustive autotuner (profiling runs)")
     ap.add_argument("--no-tuning", action="store_true", help="skip the learned-tuning evaluation")
@@ -484,6 +542,12 @@ def main():
     host_vv = vf.cpu().numpy()
     del F, A, vf, vp
     torch.cuda.empty_cache()
+    if not args.no_precision:
+        try:
+            out["precision_form"] = precision_form(M, torch, dev)
+        except Exception as e:
+            out["precision_form"] = {"error": str(e)}
+        torch.cuda.empty_cache()
     if not args.no_pdmpk:
         try:
             out["pdmpk_form"] = pdmpk_form(M, torch, dev, rp, ci, va, x, N, k, ms_plain, args.steps)
