@@ -266,13 +266,14 @@ def pdmpk_form(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps):
             "rows": st["rows"], "nnz": st["nnz"], "ghost_copies": st["ghosts"], "prep_seconds": round(t_prep, 2)}
 
 
-def precision_form(M, torch, dev, N=2048, k=16, blocks=(32, 32), s=2, steps=10):
+def precision_form(M, torch, dev, N=2048, k=16, blocks=(64, 64), s=3, blocks32=(64, 32), s32=4, steps=10):
     """The precision workload: 2D 9-point stencil 2048^2 (4,194,304 rows),
     off-diagonals -(1 + 0.1 N(0,1)) (numpy default_rng seed 5), diagonal = row
     abs-sum + 0.5, k = 16, x from SplitMix64 seed 0x5EED.  FP64 fused (bitwise
     equal to k x SpMV) is the reference; the FP32 and BF16 value streams
     accumulate in FP32 with exact power-of-two renormalisation, and their
-    error is stated per power against the FP64 block (both on the GPU)."""
+    error is stated per power against the FP64 block (both on the GPU).
+    Tiles and levels per launch from the autotuner's sweep of this matrix."""
     rp, ci, va = M.gen2("m9p", N, -1.0)
     n = len(rp) - 1
     rows = np.repeat(np.arange(n), np.diff(rp))
@@ -284,6 +285,7 @@ def precision_form(M, torch, dev, N=2048, k=16, blocks=(32, 32), s=2, steps=10):
     x = M.uniform_pm1(0x5EED, n)
     xd = torch.from_numpy(x).to(dev)
     part = M.partition_grid([N, N], list(blocks))
+    part32 = M.partition_grid([N, N], list(blocks32))
 
     def timed(fn):
         for _ in range(2):
@@ -307,18 +309,20 @@ def precision_form(M, torch, dev, N=2048, k=16, blocks=(32, 32), s=2, steps=10):
     out = {"workload": f"2D 9-pt {N}^2 (n={n}, nnz={len(ci)}), random off-diagonals, k={k}",
            "kxspmv_fp64_ms": round(ms_plain, 4),
            "fused_fp64": {"ms": round(ms64, 4), "speedup_vs_kxspmv": round(ms_plain / ms64, 3),
-                          "bitwise_equal_kxspmv": bool(torch.equal(v64, vp)), "codec": F.stats()["codec"]}}
+                          "bitwise_equal_kxspmv": bool(torch.equal(v64, vp)), "codec": F.stats()["codec"],
+                          "tile_blocks": list(blocks), "levels_per_launch": s}}
     del F, A, vp
     ref = v64.view(k + 1, n)
     for name, code in (("fp32", M.FusedMPK32.FP32), ("bf16", M.FusedMPK32.BF16)):
-        F32 = M.FusedMPK32(rp, ci, va, part, k, s, code)
+        F32 = M.FusedMPK32(rp, ci, va, part32, k, s32, code)
         v32 = torch.zeros(n * (k + 1), dtype=torch.float32, device=dev)
         ms = timed(lambda: F32.exec(xd, v32))
         scale = torch.tensor([F32.sigma ** l for l in range(k + 1)], dtype=torch.float64, device=dev)
         errs = (torch.linalg.norm(v32.view(k + 1, n).double() * scale[:, None] - ref, dim=1)
                 / torch.linalg.norm(ref, dim=1)).cpu().numpy()
         out[f"fused_{name}"] = {"ms": round(ms, 4), "speedup_vs_kxspmv_fp64": round(ms_plain / ms, 3),
-                               "sigma": F32.sigma, "per_power_rel_err_vs_fp64": [float("%.2e" % e) for e in errs]}
+                               "sigma": F32.sigma, "tile_blocks": list(blocks32), "levels_per_launch": s32,
+                               "per_power_rel_err_vs_fp64": [float("%.2e" % e) for e in errs]}
         del F32, v32
     return out
 
