@@ -325,32 +325,7 @@ int mpkx_batch_exec(mpkx_batch *b, const float *X, float *VV, void *stream, int 
     MPKX_HIP(hipGetLastError());
     return MPKX_OK;
   }
-  if (!b->graph.matches(X, VV, st, 0)) {
-    b->graph.reset();
-    hipStream_t cap = st;
-    bool own = false;
-    if (cap == nullptr) {
-      MPKX_HIP(hipStreamCreate(&cap));
-      own = true;
-    }
-    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    bool ok = batch_launches(b, X, VV, cap);
-    hipGraph_t g = nullptr;
-    hipError_t ec = hipStreamEndCapture(cap, &g);
-    if (own) (void)hipStreamDestroy(cap);
-    if (!ok || ec != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(ok ? MPKX_EHIP : MPKX_EUNSUPPORTED, "batch_exec: graph capture failed");
-    }
-    b->graph.graph = g;
-    MPKX_HIP(hipGraphInstantiate(&b->graph.exec, b->graph.graph, nullptr, nullptr, 0));
-    b->graph.k0 = X;
-    b->graph.k1 = VV;
-    b->graph.stream = st;
-    b->graph.k = 0;
-  }
-  MPKX_HIP(hipGraphLaunch(b->graph.exec, st));
-  return MPKX_OK;
+  return graph_run(b->graph, X, VV, 0, st, [&](hipStream_t s) { return batch_launches(b, X, VV, s); });
 }
 
 int mpkx_batch_stats(const mpkx_batch *b, long *rows, long *nnz, long *long_rows, int *launches) {

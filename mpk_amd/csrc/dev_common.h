@@ -65,6 +65,48 @@ struct GraphCache {
   }
 };
 
+// Records launch(stream) into gc's hipGraph for the key (a, b, key, stream)
+// and replays it; a later call with the same key only replays.  launch
+// returns false when it cannot run the configuration (nothing is kept).
+// The legacy default stream cannot be captured, so a private stream
+// records the graph in that case.
+template <class F>
+int graph_run(GraphCache &gc, const void *a, const void *b, int key, hipStream_t st, F launch) {
+  if (!gc.matches(a, b, st, key)) {
+    gc.reset();
+    hipStream_t cap = st;
+    bool own = false;
+    if (cap == nullptr) {
+      if (hipStreamCreate(&cap) != hipSuccess) return fail(MPKX_EHIP, "graph capture: hipStreamCreate failed");
+      own = true;
+    }
+    bool ok = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      ok = launch(cap);
+      hipError_t e2 = hipStreamEndCapture(cap, &g);
+      if (e2 != hipSuccess) e = e2;
+    }
+    hipGraphExec_t ge = nullptr;
+    if (e == hipSuccess && ok) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (own) (void)hipStreamDestroy(cap);
+    if (e != hipSuccess || !ok) {
+      if (g) (void)hipGraphDestroy(g);
+      return ok ? fail(MPKX_EHIP, "graph capture failed: %s", hipGetErrorString(e))
+                : fail(MPKX_EUNSUPPORTED, "graph capture: launch configuration not supported");
+    }
+    gc.graph = g;
+    gc.exec = ge;
+    gc.k0 = a;
+    gc.k1 = b;
+    gc.stream = st;
+    gc.k = key;
+  }
+  MPKX_HIP(hipGraphLaunch(gc.exec, st));
+  return MPKX_OK;
+}
+
 // Workgroup id -> work item so that consecutive items land on the same XCD
 // (the dispatcher hands workgroup i to XCD i % 8; each XCD has its own L2).
 __device__ __forceinline__ int xcd_remap(int i, int count) {

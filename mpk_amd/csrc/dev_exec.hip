@@ -432,25 +432,10 @@ int mpkx_spmv_k(mpkx_csr *m, double *vv, int k, void *stream, int use_graph) {
     MPKX_HIP(hipGetLastError());
     return MPKX_OK;
   }
-  if (!m->graph.matches(vv, nullptr, st, k)) {
-    m->graph.reset();
-    hipStream_t cap = st;
-    bool own = false;
-    if (cap == nullptr) {  // the legacy default stream cannot be captured
-      MPKX_HIP(hipStreamCreate(&cap));
-      own = true;
-    }
-    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    spmv_launches(m, vv, k, cap);
-    MPKX_HIP(hipStreamEndCapture(cap, &m->graph.graph));
-    MPKX_HIP(hipGraphInstantiate(&m->graph.exec, m->graph.graph, nullptr, nullptr, 0));
-    if (own) MPKX_HIP(hipStreamDestroy(cap));
-    m->graph.k0 = vv;
-    m->graph.stream = st;
-    m->graph.k = k;
-  }
-  MPKX_HIP(hipGraphLaunch(m->graph.exec, st));
-  return MPKX_OK;
+  return graph_run(m->graph, vv, nullptr, k, st, [&](hipStream_t s) {
+    spmv_launches(m, vv, k, s);
+    return true;
+  });
 }
 
 }  // extern "C"

@@ -103,7 +103,10 @@ int mpkx_fused_exec(mpkx_fused *f, const double *x, double *vv, void *stream, in
     MPKX_HIP(hipGetLastError());
     return MPKX_OK;
   }
-  return graph_run(f->graph, x, vv, st, [&](hipStream_t s) { fused_launches(f, x, vv, s); });
+  return graph_run(f->graph, x, vv, 0, st, [&](hipStream_t s) {
+    fused_launches(f, x, vv, s);
+    return true;
+  });
 }
 
 int mpkx_fused_round(mpkx_fused *f, double *vv, int round, void *stream) {

@@ -163,7 +163,10 @@ int mpkx_fused32_exec(mpkx_fused32 *f, const double *x, float *vv, void *stream,
     MPKX_HIP(hipGetLastError());
     return MPKX_OK;
   }
-  return graph_run(f->graph, x, vv, st, [&](hipStream_t s) { fused32_launches(f, x, vv, s); });
+  return graph_run(f->graph, x, vv, 0, st, [&](hipStream_t s) {
+    fused32_launches(f, x, vv, s);
+    return true;
+  });
 }
 
 int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream) {

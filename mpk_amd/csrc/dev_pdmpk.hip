@@ -248,26 +248,10 @@ int mpkx_pd_exec(mpkx_pd_plan *pl, const double *x, double *y, int send_home, vo
     MPKX_HIP(hipGetLastError());
     return MPKX_OK;
   }
-  if (!pl->graph.matches(x, y, st, send_home != 0)) {
-    pl->graph.reset();
-    hipStream_t cap = st;
-    bool own = false;
-    if (cap == nullptr) {  // the legacy default stream cannot be captured
-      MPKX_HIP(hipStreamCreate(&cap));
-      own = true;
-    }
-    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    pd_launches(pl, x, y, send_home, cap);
-    MPKX_HIP(hipStreamEndCapture(cap, &pl->graph.graph));
-    MPKX_HIP(hipGraphInstantiate(&pl->graph.exec, pl->graph.graph, nullptr, nullptr, 0));
-    if (own) MPKX_HIP(hipStreamDestroy(cap));
-    pl->graph.k0 = x;
-    pl->graph.k1 = y;
-    pl->graph.stream = st;
-    pl->graph.k = send_home != 0;
-  }
-  MPKX_HIP(hipGraphLaunch(pl->graph.exec, st));
-  return MPKX_OK;
+  return graph_run(pl->graph, x, y, send_home != 0, st, [&](hipStream_t s) {
+    pd_launches(pl, x, y, send_home, s);
+    return true;
+  });
 }
 
 }  // extern "C"

@@ -501,29 +501,4 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   return MPKX_OK;
 }
 
-// Records `launch(stream)` into a hipGraph keyed by (a, b, stream) and replays it.
-template <class F>
-int graph_run(GraphCache &gc, const void *a, const void *b, hipStream_t st, F launch) {
-  if (!gc.matches(a, b, st, 0)) {
-    gc.reset();
-    hipStream_t cap = st;
-    bool own = false;
-    if (cap == nullptr) {  // the legacy default stream cannot be captured
-      MPKX_HIP(hipStreamCreate(&cap));
-      own = true;
-    }
-    MPKX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    launch(cap);
-    MPKX_HIP(hipStreamEndCapture(cap, &gc.graph));
-    MPKX_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
-    if (own) MPKX_HIP(hipStreamDestroy(cap));
-    gc.k0 = a;
-    gc.k1 = b;
-    gc.stream = st;
-    gc.k = 0;
-  }
-  MPKX_HIP(hipGraphLaunch(gc.exec, st));
-  return MPKX_OK;
-}
-
 }  // namespace mpkx
