@@ -128,29 +128,9 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   const int base = h[FUSED_HDR - 1];
   const int tid = threadIdx.x;
 
-  // gather: GB slots per thread in flight at a time (row indices, then
-  // values, then LDS stores) so each wave keeps several loads outstanding
-  constexpr int GB = 8;
-  const int nin = cnt[w];
-  const VT *src = vv + (size_t)L0 * n;
-  for (int t0 = tid; t0 < nin; t0 += GB * T) {
-    int gi[GB];
-    VT gv[GB];
-#pragma unroll
-    for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] + base : -1;
-#pragma unroll
-    for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
-#pragma unroll
-    for (int u = 0; u < GB; u++)
-      if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, maxslots + 1, gv[u], d0, d1, d2, d3);
-  }
-  if (tid == 0) {
-    put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
-    put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
-  }
-
   // matrix rows of this thread into registers (W2 consecutive words per
-  // row: one wide load), own-row targets
+  // row: one wide load) and own-row targets, issued first so that their
+  // latency overlaps the gather
   const int ncomp = cnt[w - 1], nown = cnt[0];
   uint32_t e[MR][W2];
   int own[MR];
@@ -178,6 +158,28 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     }
     own[m] = r < nown ? g[r] + base : -1;
   }
+
+  // gather: GB slots per thread in flight at a time (row indices, then
+  // values, then LDS stores) so each wave keeps several loads outstanding
+  constexpr int GB = 8;
+  const int nin = cnt[w];
+  const VT *src = vv + (size_t)L0 * n;
+  for (int t0 = tid; t0 < nin; t0 += GB * T) {
+    int gi[GB];
+    VT gv[GB];
+#pragma unroll
+    for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] + base : -1;
+#pragma unroll
+    for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+#pragma unroll
+    for (int u = 0; u < GB; u++)
+      if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, maxslots + 1, gv[u], d0, d1, d2, d3);
+  }
+  if (tid == 0) {
+    put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
+    put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
+  }
+
   __syncthreads();
 
   VT *in = buf0, *out = buf1;
