@@ -27,6 +27,7 @@ FusedKernel<float, MT> pick32(int T, int MR, int W, int nd) {
   if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4>(W, nd);
   if (T == 1024 && MR == 4) return pick_fused_w<float, MT, 1024, 4>(W, nd);
   if (T == 1024 && MR == 8) return pick_fused_w<float, MT, 1024, 8>(W, nd);
+  if (T == 1024 && MR == 12) return pick_fused_w<float, MT, 1024, 12>(W, nd);
   return nullptr;
 }
 
@@ -92,7 +93,7 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
   int rc;
   try {
     rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, dict_ok, 2 * (int)sizeof(float),
-                         {{512, 4}, {1024, 4}, {1024, 8}}, f->plan, dict_codec);
+                         {{512, 4}, {1024, 4}, {1024, 8}, {1024, 12}}, f->plan, dict_codec);
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fused32_create: host allocation");
   }

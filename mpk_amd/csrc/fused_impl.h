@@ -243,6 +243,8 @@ using FusedKernel = void (*)(const int *, const int *, const uint32_t *, const M
 
 template <typename VT, typename MT, int T, int MR, int W>
 FusedKernel<VT, MT> pick_fused_nd(int nd) {
+  if constexpr (MR * ((W + 1) / 2) > 64)  // beyond the plan builder's register budget
+    return nullptr;
   if constexpr (std::is_same<MT, uint16_t>::value)  // BF16 values always stream
     return nd == 0 ? k_fused<VT, MT, T, MR, W, 0> : nullptr;
   switch (nd) {
