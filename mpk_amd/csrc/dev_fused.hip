@@ -9,11 +9,6 @@ using namespace mpkx;
 
 namespace {
 
-__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long n) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] = src[t];
-}
-
 using K64 = FusedKernel<double, double>;
 
 K64 pick64(int T, int MR, int W, int nd) {
@@ -37,18 +32,18 @@ struct mpkx_fused {
   GraphCache graph;
 };
 
-static void launch_round(mpkx_fused *f, double *vv, int L0, hipStream_t st) {
+// xin: x for the first round of an exec (level 0 comes from x and the
+// tiles store it into vv), else null (level L0 is already in vv).
+static void launch_round(mpkx_fused *f, double *vv, int L0, const double *xin, hipStream_t st) {
   const TilePlan &P = f->plan;
   int w = std::min(P.s, P.nlevel - L0);
   hipLaunchKernelGGL(f->fn, dim3(P.ntiles), dim3(P.T), f->lds_bytes, st, f->hdr.p, f->grow.p, f->ent.p,
-                     f->vals.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], 1.0, vv, P.n, L0, w, P.ntiles,
+                     f->vals.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], 1.0, vv, xin, P.n, L0, w, P.ntiles,
                      P.maxslots);
 }
 
 static void fused_launches(mpkx_fused *f, const double *x, double *vv, hipStream_t st) {
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)((f->plan.n + 255) / 256)), dim3(256), 0, st, vv, x,
-                     (long)f->plan.n);
-  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s) launch_round(f, vv, L0, st);
+  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s) launch_round(f, vv, L0, L0 == 0 ? x : nullptr, st);
 }
 
 extern "C" {
@@ -113,7 +108,7 @@ int mpkx_fused_round(mpkx_fused *f, double *vv, int round, void *stream) {
   clear_error();
   if (!f || !vv || round < 0 || (long)round * f->plan.s >= f->plan.nlevel)
     return fail(MPKX_EINVAL, "fused_round: bad arguments");
-  launch_round(f, vv, round * f->plan.s, (hipStream_t)stream);
+  launch_round(f, vv, round * f->plan.s, nullptr, (hipStream_t)stream);
   MPKX_HIP(hipGetLastError());
   return MPKX_OK;
 }

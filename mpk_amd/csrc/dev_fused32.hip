@@ -14,11 +14,6 @@ using namespace mpkx;
 
 namespace {
 
-__global__ void k_to_f32(float *__restrict__ dst, const double *__restrict__ src, long n) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] = (float)src[t];
-}
-
 using K32f = FusedKernel<float, float>;
 using K32b = FusedKernel<float, uint16_t>;
 
@@ -59,23 +54,23 @@ struct mpkx_fused32 {
   GraphCache graph;
 };
 
-static void launch_round32(mpkx_fused32 *f, float *vv, int L0, hipStream_t st) {
+// xin: the FP64 x for the first round of an exec (rounded to FP32 as it is
+// gathered; the tiles store level 0 into vv), else null.
+static void launch_round32(mpkx_fused32 *f, float *vv, int L0, const double *xin, hipStream_t st) {
   const TilePlan &P = f->plan;
   int w = std::min(P.s, P.nlevel - L0);
   if (f->values == 2)
     hipLaunchKernelGGL(f->fnb, dim3(P.ntiles), dim3(P.T), f->lds_bytes, st, f->hdr.p, f->grow.p, f->ent.p,
-                       f->valb.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], f->inv_sigma, vv, P.n, L0, w,
+                       f->valb.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], f->inv_sigma, vv, xin, P.n, L0, w,
                        P.ntiles, P.maxslots);
   else
     hipLaunchKernelGGL(f->fnf, dim3(P.ntiles), dim3(P.T), f->lds_bytes, st, f->hdr.p, f->grow.p, f->ent.p,
-                       f->valf.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], f->inv_sigma, vv, P.n, L0, w,
+                       f->valf.p, f->dv[0], f->dv[1], f->dv[2], f->dv[3], f->inv_sigma, vv, xin, P.n, L0, w,
                        P.ntiles, P.maxslots);
 }
 
 static void fused32_launches(mpkx_fused32 *f, const double *x, float *vv, hipStream_t st) {
-  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((f->plan.n + 255) / 256)), dim3(256), 0, st, vv, x,
-                     (long)f->plan.n);
-  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s) launch_round32(f, vv, L0, st);
+  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s) launch_round32(f, vv, L0, L0 == 0 ? x : nullptr, st);
 }
 
 extern "C" {
@@ -173,7 +168,7 @@ int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream) {
   clear_error();
   if (!f || !vv || round < 0 || (long)round * f->plan.s >= f->plan.nlevel)
     return fail(MPKX_EINVAL, "fused32_round: bad arguments");
-  launch_round32(f, vv, round * f->plan.s, (hipStream_t)stream);
+  launch_round32(f, vv, round * f->plan.s, nullptr, (hipStream_t)stream);
   MPKX_HIP(hipGetLastError());
   return MPKX_OK;
 }

@@ -112,8 +112,9 @@ __device__ __forceinline__ void put_slot(VT *buf, int slot, int plane, VT v, VT 
 template <typename VT, typename MT, int T, int MR, int W, int ND>
 __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
                                              const uint32_t *__restrict__ ent, const MT *__restrict__ vals,
-                                             VT d0, VT d1, VT d2, VT d3, VT inv_sigma, VT *vv, int n,
-                                             int L0, int w, int ntiles, int maxslots) {
+                                             VT d0, VT d1, VT d2, VT d3, VT inv_sigma, VT *vv,
+                                             const double *__restrict__ xin, int n, int L0, int w, int ntiles,
+                                             int maxslots) {
   constexpr int W2 = (W + 1) / 2;
   constexpr bool SCALE = std::is_same<VT, float>::value;
   constexpr int NP = ND > 0 ? ND : 1;  // LDS words per slot
@@ -160,7 +161,9 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   }
 
   // gather: GB slots per thread in flight at a time (row indices, then
-  // values, then LDS stores) so each wave keeps several loads outstanding
+  // values, then LDS stores) so each wave keeps several loads outstanding.
+  // With xin (the first round of an exec) level 0 is read from x itself and
+  // the tile writes its own rows of level 0 into vv, replacing a copy pass.
   constexpr int GB = 8;
   const int nin = cnt[w];
   const VT *src = vv + (size_t)L0 * n;
@@ -169,8 +172,16 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     VT gv[GB];
 #pragma unroll
     for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] + base : -1;
+    if (xin) {
 #pragma unroll
-    for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+      for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? (VT)xin[gi[u]] : (VT)0;
+#pragma unroll
+      for (int u = 0; u < GB; u++)
+        if (gi[u] >= 0 && t0 + u * T < nown) vv[gi[u]] = gv[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+    }
 #pragma unroll
     for (int u = 0; u < GB; u++)
       if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, maxslots + 1, gv[u], d0, d1, d2, d3);
@@ -239,7 +250,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
 
 template <typename VT, typename MT>
 using FusedKernel = void (*)(const int *, const int *, const uint32_t *, const MT *, VT, VT, VT, VT, VT,
-                             VT *, int, int, int, int, int);
+                             VT *, const double *, int, int, int, int, int);
 
 template <typename VT, typename MT, int T, int MR, int W>
 FusedKernel<VT, MT> pick_fused_nd(int nd) {
