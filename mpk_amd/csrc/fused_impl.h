@@ -172,15 +172,23 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
     VT gv[GB];
 #pragma unroll
     for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] + base : -1;
-    if (xin) {
+    if constexpr (std::is_same<VT, double>::value) {  // one load path: pick the source array
+      const double *sp = xin ? xin : src;
+#pragma unroll
+      for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? sp[gi[u]] : 0.0;
+    } else if (xin) {
 #pragma unroll
       for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? (VT)xin[gi[u]] : (VT)0;
-#pragma unroll
-      for (int u = 0; u < GB; u++)
-        if (gi[u] >= 0 && t0 + u * T < nown) vv[gi[u]] = gv[u];
     } else {
 #pragma unroll
       for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+    }
+    if constexpr (ND > 0) {  // product records do not keep x in LDS: store level 0 here
+      if (xin) {
+#pragma unroll
+        for (int u = 0; u < GB; u++)
+          if (gi[u] >= 0 && t0 + u * T < nown) vv[gi[u]] = gv[u];
+      }
     }
 #pragma unroll
     for (int u = 0; u < GB; u++)
@@ -189,6 +197,14 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   if (tid == 0) {
     put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
     put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
+  }
+  if constexpr (ND <= 0) {  // LDS holds x: the own rows of level 0 go out from there
+    if (xin) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < MR; m++)
+        if (own[m] >= 0) vv[own[m]] = buf0[m * T + tid];
+    }
   }
 
   __syncthreads();
@@ -254,19 +270,21 @@ using FusedKernel = void (*)(const int *, const int *, const uint32_t *, const M
 
 template <typename VT, typename MT, int T, int MR, int W>
 FusedKernel<VT, MT> pick_fused_nd(int nd) {
-  if constexpr (MR * ((W + 1) / 2) > 64)  // beyond the plan builder's register budget
+  if constexpr (MR * ((W + 1) / 2) > 64) {  // beyond the plan builder's register budget
     return nullptr;
-  if constexpr (std::is_same<MT, uint16_t>::value)  // BF16 values always stream
+  } else if constexpr (std::is_same<MT, uint16_t>::value) {  // BF16 values always stream
     return nd == 0 ? k_fused<VT, MT, T, MR, W, 0> : nullptr;
-  switch (nd) {
-    case 0: return k_fused<VT, MT, T, MR, W, 0>;
-    case 1: return k_fused<VT, MT, T, MR, W, 1>;
-    case 2: return k_fused<VT, MT, T, MR, W, 2>;
-    case 4: return k_fused<VT, MT, T, MR, W, 4>;
-    case -2: return k_fused<VT, MT, T, MR, W, -2>;
-    case -4: return k_fused<VT, MT, T, MR, W, -4>;
+  } else {
+    switch (nd) {
+      case 0: return k_fused<VT, MT, T, MR, W, 0>;
+      case 1: return k_fused<VT, MT, T, MR, W, 1>;
+      case 2: return k_fused<VT, MT, T, MR, W, 2>;
+      case 4: return k_fused<VT, MT, T, MR, W, 4>;
+      case -2: return k_fused<VT, MT, T, MR, W, -2>;
+      case -4: return k_fused<VT, MT, T, MR, W, -4>;
+    }
+    return nullptr;
   }
-  return nullptr;
 }
 
 // nd: 0 value stream, 1/2/4 product records, -2/-4 select codec.
