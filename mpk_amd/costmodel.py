@@ -6,7 +6,7 @@ lanes splitting each row's entries, and the long-row threshold T above which
 a row gets a workgroup.
 
 * `space(nrhs)` lists every valid configuration; `tune` times all of them on
-  the GPU (the exhaustive autotuner, which is also the judge of any pick).
+  the GPU (the exhaustive autotuner, against which every pick is measured).
 * `features` (in the package) describes a matrix by 24 numbers.
 * `CostModel` is a 24 -> 64 -> 64 -> 3 MLP whose three heads predict
   log2 B, log2 L and log2 T of the fastest configuration.  It is trained on

@@ -3,8 +3,8 @@
 For a matrix and k it times every candidate (partition, levels per launch s,
 precision) on the GPU and returns the measurements sorted by time.  A
 candidate whose tile region does not fit the LDS or register budget is
-reported as unsupported.  This is the judge that any learned cost model is
-checked against.
+reported as unsupported.  Every learned cost model is measured against this
+sweep.
 
 Candidates for grid matrices are Cartesian block partitions (tile shapes);
 for general matrices the graph partitioner with a range of part counts.
