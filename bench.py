@@ -327,6 +327,41 @@ def precision_form(M, torch, dev, N=2048, k=16, blocks=(64, 64), s=3, blocks32=(
     return out
 
 
+def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
+                   cases=((64, 0), (64, 1), (64, 2), (64, 3), (512, 0), (512, 3))):
+    """The reference's own mpk2 schedule on the GPU (mpkx_plan: the phases'
+    partition-local rows and ghost copies, bitwise equal to the reference
+    exec) at 128^3 for P partitions (block partitions, each phase's shifted
+    by half a tile) and nphase = 0..3, against k x SpMV."""
+    out = []
+    xd = torch.from_numpy(x).to(dev)
+    for P, nphase in cases:
+        b = round(P ** (1 / 3))
+        blocks = [b, b, b]
+        t0 = time.perf_counter()
+        parts = [M.partition_grid([N] * 3, blocks, [(ph * N // b) // 2] * 3) for ph in range(max(nphase, 1))]
+        _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
+        t_prep = time.perf_counter() - t0
+        plan = M.Plan(bufs)
+        vv = torch.zeros(len(x) * (k + 1), dtype=torch.float64, device=dev)
+        for _ in range(2):
+            plan.exec(xd, vv)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            plan.exec(xd, vv)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        st = plan.stats()
+        out.append({"npart": P, "nphase": nphase, "ms_per_step": round(ms, 5), "vs_kxspmv": round(ms_plain / ms, 3),
+                    "redundancy": round(st["rows"] / (len(x) * k), 3), "ghosts": st["ghosts"],
+                    "prep_seconds": round(t_prep, 2)})
+        del plan, bufs, vv
+    return out
+
+
 def rocsparse_kxspmv(torch, rp, ci, va, xd, k, steps):
     """k back-to-back SpMVs through torch.sparse CSR (hipSPARSE/rocSPARSE on
     ROCm), as an outside reference point for the plain kernel."""
@@ -371,6 +406,7 @@ def main():
     ap.add_argument("--no-sstep", action="store_true", help="skip the s-step CG consumer")
     ap.add_argument("--no-pdmpk", action="store_true", help="skip the pdmpk schedule")
     ap.add_argument("--no-precision", action="store_true", help="skip the 9-point FP64/FP32/BF16 form")
+    ap.add_argument("--no-exact", action="store_true", help="skip the reference mpk2 schedule on the GPU")
     ap.add_argument("--batch-config", default=None,
                     help="B,L,T for the batched form instead of the exhaustive autotuner (profiling runs)")
     ap.add_argument("--no-tuning", action="store_true", help="skip the learned-tuning evaluation")
@@ -546,6 +582,12 @@ def main():
     host_vv = vf.cpu().numpy()
     del F, A, vf, vp
     torch.cuda.empty_cache()
+    if not args.no_exact:
+        try:
+            out["exact_schedule"] = exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, args.steps)
+        except Exception as e:
+            out["exact_schedule"] = {"error": str(e)}
+        torch.cuda.empty_cache()
     if not args.no_precision:
         try:
             out["precision_form"] = precision_form(M, torch, dev)

@@ -202,8 +202,10 @@ void mpkx_plan_destroy(mpkx_plan *plan);
 int mpkx_exec_bufs(mpkx_plan *plan, mpkx_buffers *const *bufs);
 /* Device form: x (device, n doubles) -> vv (device, n*(nlevel+1)),
  * loading rbuf[0] = x[idx_buf] and collecting like collect_results
- * (mpk2/mpi2_mpkexec.c:122-146). */
-int mpkx_exec(mpkx_plan *plan, const double *x, double *vv, void *stream);
+ * (mpk2/mpi2_mpkexec.c:122-146).  use_graph != 0 records the launches
+ * once per (x, vv, stream) into a hipGraph and replays it. */
+int mpkx_exec(mpkx_plan *plan, const double *x, double *vv, void *stream,
+              int use_graph);
 /* Device-to-device copy of n doubles by a plain streaming kernel
  * (k_dcopy: one load and one store per element).  Used to calibrate HBM
  * traffic counters against a kernel of known size. */

@@ -93,7 +93,7 @@ def lib():
         "mpkx_plan_create": ([P, P, C.c_int], C.c_int),
         "mpkx_plan_destroy": ([P], None),
         "mpkx_exec_bufs": ([P, P], C.c_int),
-        "mpkx_exec": ([P, P, P, P], C.c_int),
+        "mpkx_exec": ([P, P, P, P, C.c_int], C.c_int),
         "mpkx_plan_stats": ([P, P, P, P, P], C.c_int),
         "mpkx_dcopy": ([P, P, C.c_long, P], C.c_int),
         "mpkx_pd_prep": ([P, C.c_int, P, P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_int], C.c_int),
@@ -477,8 +477,8 @@ class Plan:
     def exec_bufs(self, bufset):
         _check(lib().mpkx_exec_bufs(self._h, bufset.array()))
 
-    def exec(self, x, vv, stream=None):
-        _check(lib().mpkx_exec(self._h, _dev(x), _dev(vv), _stream(stream)))
+    def exec(self, x, vv, stream=None, use_graph=True):
+        _check(lib().mpkx_exec(self._h, _dev(x), _dev(vv), _stream(stream), int(use_graph)))
 
     def stats(self):
         r, z, g, l = C.c_long(), C.c_long(), C.c_long(), C.c_int()

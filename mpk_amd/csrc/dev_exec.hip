@@ -126,6 +126,7 @@ struct mpkx_plan {
   DevArray<int> ci, out_row, blk, gdst, gsrc, xdst, xsrc, csrc;
   DevArray<double> va, vv, stage;
   long rows = 0, nnz = 0, ghosts = 0;
+  GraphCache graph;
 };
 
 extern "C" {
@@ -357,10 +358,10 @@ int mpkx_exec_bufs(mpkx_plan *pl, mpkx_buffers *const *bufs) {
   return MPKX_OK;
 }
 
-int mpkx_exec(mpkx_plan *pl, const double *x, double *vv, void *stream) {
-  clear_error();
-  if (!pl || !x || !vv) return fail(MPKX_EINVAL, "exec: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
+}  // extern "C"
+
+// Input load, every phase, result collection (no host synchronisation).
+static int exec_launches(mpkx_plan *pl, const double *x, double *vv, hipStream_t st) {
   long nx = (long)pl->xdst.count;
   // rbuf[0] of every partition = x[idx_buf[i]]
   if (nx > 0)
@@ -370,8 +371,22 @@ int mpkx_exec(mpkx_plan *pl, const double *x, double *vv, void *stream) {
   if (rc) return rc;
   long nv = (long)pl->csrc.count;
   hipLaunchKernelGGL(k_gather, dim3(grid1(nv, 256)), dim3(256), 0, st, vv, pl->vv.p, pl->csrc.p, nv);
-  MPKX_HIP(hipGetLastError());
   return MPKX_OK;
+}
+
+extern "C" {
+
+int mpkx_exec(mpkx_plan *pl, const double *x, double *vv, void *stream, int use_graph) {
+  clear_error();
+  if (!pl || !x || !vv) return fail(MPKX_EINVAL, "exec: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    int rc = exec_launches(pl, x, vv, st);
+    if (rc) return rc;
+    MPKX_HIP(hipGetLastError());
+    return MPKX_OK;
+  }
+  return graph_run(pl->graph, x, vv, 0, st, [&](hipStream_t s) { return exec_launches(pl, x, vv, s) == MPKX_OK; });
 }
 
 // ---------------------------------------------------------------------------

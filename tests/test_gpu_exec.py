@@ -193,9 +193,10 @@ def test_exact_plan_cube64_against_oracle():
         parts = [M.partition_grid([64] * 3, [2, 2, 2], [16 * ph] * 3) for ph in range(max(nphase, 1))]
         _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
         plan = M.Plan(bufs)
-        vv = zeros_dev(n * (k + 1))
-        plan.exec(to_dev(x), vv)
-        assert np.array_equal(to_host(vv), gold)
+        for graph in (False, True):
+            vv = zeros_dev(n * (k + 1))
+            plan.exec(to_dev(x), vv, use_graph=graph)
+            assert np.array_equal(to_host(vv), gold), (nphase, graph)
 
 
 def _rel_errs_scaled(out32, gold, n, k, sigma):
