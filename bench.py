@@ -355,10 +355,27 @@ def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
         st = plan.stats()
-        out.append({"npart": P, "nphase": nphase, "ms_per_step": round(ms, 5), "vs_kxspmv": round(ms_plain / ms, 3),
-                    "redundancy": round(st["rows"] / (len(x) * k), 3), "ghosts": st["ghosts"],
-                    "prep_seconds": round(t_prep, 2)})
-        del plan, bufs, vv
+        rec = {"npart": P, "nphase": nphase, "ms_per_step": round(ms, 5), "vs_kxspmv": round(ms_plain / ms, 3),
+               "redundancy": round(st["rows"] / (len(x) * k), 3), "ghosts": st["ghosts"],
+               "prep_seconds": round(t_prep, 2)}
+        del plan, bufs
+        if P == 64 and nphase <= 1:  # the shared-memory task lists of the same schedule (mpk2/mpkprep.c)
+            levels, sk, _, _ = M.prepare(rp, ci, va, k, nphase, parts)
+            tcount, tidx = M.prep_mpk(rp, ci, k, nphase, parts, levels, sk)
+            T = M.TaskPlan(rp, ci, va, k, nphase, P, tcount, tidx)
+            T.exec(xd, vv)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(steps):
+                T.exec(xd, vv)
+            e1.record()
+            torch.cuda.synchronize()
+            tms = e0.elapsed_time(e1) / steps
+            rec["task_lists"] = {"ms_per_step": round(tms, 5), "vs_kxspmv": round(ms_plain / tms, 3),
+                                 "tasks": int(len(tidx))}
+            del T
+        out.append(rec)
+        del vv
     return out
 
 
