@@ -340,7 +340,7 @@ def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
         blocks = [b, b, b]
         t0 = time.perf_counter()
         parts = [M.partition_grid([N] * 3, blocks, [(ph * N // b) // 2] * 3) for ph in range(max(nphase, 1))]
-        _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
+        levels, sk, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
         t_prep = time.perf_counter() - t0
         plan = M.Plan(bufs)
         vv = torch.zeros(len(x) * (k + 1), dtype=torch.float64, device=dev)
@@ -360,7 +360,6 @@ def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
                "prep_seconds": round(t_prep, 2)}
         del plan, bufs
         if P == 64 and nphase <= 1:  # the shared-memory task lists of the same schedule (mpk2/mpkprep.c)
-            levels, sk, _, _ = M.prepare(rp, ci, va, k, nphase, parts)
             tcount, tidx = M.prep_mpk(rp, ci, k, nphase, parts, levels, sk)
             T = M.TaskPlan(rp, ci, va, k, nphase, P, tcount, tidx)
             T.exec(xd, vv)
