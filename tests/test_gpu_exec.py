@@ -182,6 +182,23 @@ def test_headline_size_128_cube_bitwise_oracle():
     assert np.array_equal(out, gold)
 
 
+def test_exact_plan_headline_size_bitwise_oracle():
+    """The reference mpk2 schedule at 128^3 with 64 partitions and 3 phases
+    (buffers from this library's prep) equals k x SpMV bitwise."""
+    say("exact plan 128^3 P=64 nphase=2")
+    N, k, nphase = 128, 8, 2
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    n = len(rp) - 1
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    parts = [M.partition_grid([N] * 3, [4, 4, 4], [16 * ph] * 3) for ph in range(nphase)]
+    _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
+    plan = M.Plan(bufs)
+    vv = zeros_dev(n * (k + 1))
+    plan.exec(to_dev(x), vv)
+    assert np.array_equal(to_host(vv), gold)
+
+
 def test_exact_plan_cube64_against_oracle():
     say("exact plan cube64")
     rp, ci, va = M.gen2("cube", 64, -1.0)
