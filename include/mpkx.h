@@ -258,6 +258,8 @@ enum {
   MPKX_FUSED_EXPLICIT = 1, /* force explicit FP64 values                  */
   MPKX_FUSED_PRODUCTS = 2, /* dictionary: product records in LDS           */
   MPKX_FUSED_SELECT = 4,   /* dictionary: per-entry select                */
+  MPKX_FUSED_POSITIONAL = 8, /* one value per ELL position (fails when the */
+                             /* matrix values are not constant per position) */
 };
 /* part[n] labels in [0, npart); s = levels per launch (1..8). */
 int mpkx_fused_create(mpkx_fused **f, int n, const int *ptr, const int *col,
@@ -272,8 +274,10 @@ int mpkx_fused_exec(mpkx_fused *f, const double *x, double *vv, void *stream,
 int mpkx_fused_round(mpkx_fused *f, double *vv, int round, void *stream);
 /* Plan statistics: max slots per tile, total slots, computed row-levels
  * per exec, matrix bytes, codec (0 explicit FP64, 1 dictionary with
- * product records in LDS, 2 dictionary with per-entry select), launches
- * per exec. */
+ * product records in LDS, 2 dictionary with per-entry select, 3
+ * positional values), launches per exec.  AUTO picks positional values
+ * when every row holds the same value at each ELL position (stencils with
+ * constant coefficients), else a dictionary codec by LDS residency. */
 int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots,
                      long *row_levels, long *matrix_bytes, int *codec,
                      int *launches);
@@ -327,11 +331,15 @@ int mpkx_pd_plan_stats(const mpkx_pd_plan *plan, int *nbatch, long *rows,
 /*   vv[l*n + i] = (A^l x)_i / sigma^l.                                 */
 /* ------------------------------------------------------------------ */
 typedef struct mpkx_fused32 mpkx_fused32;
-/* AUTO: dictionary when the matrix has <= 4 distinct values (codec picked */
-/* by LDS residency), else FP32; *_PRODUCTS / *_SELECT force the codec   */
-/* (product records fall back to select when they do not fit the LDS).  */
+/* AUTO: positional values when every row holds the same value at each  */
+/* ELL position, else a dictionary when the matrix has <= 4 distinct     */
+/* values (codec picked by LDS residency), else FP32; *_PRODUCTS /       */
+/* *_SELECT / *_POSITIONAL force the codec (product records fall back to */
+/* select when they do not fit the LDS; positional fails when the matrix */
+/* does not allow it).                                                   */
 enum { MPKX_VALUES_AUTO = 0, MPKX_VALUES_FP32 = 1, MPKX_VALUES_BF16 = 2,
-       MPKX_VALUES_AUTO_PRODUCTS = 3, MPKX_VALUES_AUTO_SELECT = 4 };
+       MPKX_VALUES_AUTO_PRODUCTS = 3, MPKX_VALUES_AUTO_SELECT = 4,
+       MPKX_VALUES_AUTO_POSITIONAL = 5 };
 int mpkx_fused32_create(mpkx_fused32 **f, int n, const int *ptr, const int *col,
                         const double *val, const int *part, int npart,
                         int nlevel, int s, int values, double *sigma);
@@ -341,7 +349,7 @@ int mpkx_fused32_exec(mpkx_fused32 *f, const double *x, float *vv,
                       void *stream, int use_graph);
 int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream);
 /* values: 0 dictionary (product records), 1 FP32 stream, 2 BF16 stream,
- * 3 dictionary (per-entry select). */
+ * 3 dictionary (per-entry select), 4 positional values. */
 int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots,
                        long *row_levels, long *matrix_bytes, int *values,
                        int *launches);

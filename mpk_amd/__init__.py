@@ -591,7 +591,7 @@ class CsrMatrix:
 class FusedMPK:
     """Fused skirt-round MPK: s powers per launch, tiles from `part`."""
 
-    EXPLICIT, PRODUCTS, SELECT = 1, 2, 4
+    EXPLICIT, PRODUCTS, SELECT, POSITIONAL = 1, 2, 4, 8
 
     def __init__(self, ptr, col, val, part, nlevel, s, flags=0):
         self._h = C.c_void_p()
@@ -617,8 +617,8 @@ class FusedMPK:
         _check(lib().mpkx_fused_stats(self._h, C.byref(ms), C.byref(sl), C.byref(rl), C.byref(mb),
                                       C.byref(codec), C.byref(la)))
         return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
-                "matrix_bytes": mb.value, "codec": ["explicit", "dict", "dict"][codec.value],
-                "dict_codec": [None, "products", "select"][codec.value],
+                "matrix_bytes": mb.value, "codec": ["explicit", "dict", "dict", "dict"][codec.value],
+                "dict_codec": [None, "products", "select", "positional"][codec.value],
                 "launches": la.value}
 
     def __del__(self, _release=_release):
@@ -647,7 +647,7 @@ class FusedMPK32:
     a dictionary (AUTO), FP32 or BF16; vv[l] = A^l x / sigma^l (exact power-of-two
     scaling per power)."""
 
-    AUTO, FP32, BF16, AUTO_PRODUCTS, AUTO_SELECT = 0, 1, 2, 3, 4
+    AUTO, FP32, BF16, AUTO_PRODUCTS, AUTO_SELECT, AUTO_POSITIONAL = 0, 1, 2, 3, 4, 5
 
     def __init__(self, ptr, col, val, part, nlevel, s, values=0):
         self._h = C.c_void_p()
@@ -674,8 +674,8 @@ class FusedMPK32:
         _check(lib().mpkx_fused32_stats(self._h, C.byref(ms), C.byref(sl), C.byref(rl), C.byref(mb),
                                         C.byref(vals), C.byref(la)))
         return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
-                "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16", "dict"][vals.value],
-                "dict_codec": {0: "products", 3: "select"}.get(vals.value),
+                "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16", "dict", "dict"][vals.value],
+                "dict_codec": {0: "products", 3: "select", 4: "positional"}.get(vals.value),
                 "launches": la.value, "sigma": self.sigma}
 
     def __del__(self, _release=_release):

@@ -59,7 +59,8 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   try {
     rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, !(flags & MPKX_FUSED_EXPLICIT),
                          2 * (int)sizeof(double), {{512, 4}, {1024, 4}, {1024, 8}}, f->plan,
-                         (flags & MPKX_FUSED_PRODUCTS) ? 1 : (flags & MPKX_FUSED_SELECT) ? 2 : 0);
+                         (flags & MPKX_FUSED_PRODUCTS) ? 1 : (flags & MPKX_FUSED_SELECT) ? 2
+                         : (flags & MPKX_FUSED_POSITIONAL) ? 3 : 0);
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fused_create: host allocation");
   }
@@ -68,7 +69,7 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   for (size_t i = 0; i < P.dictv.size(); i++) f->dv[i] = P.dictv[i];
   f->fn = pick64(P.T, P.MR, P.W, P.ndict);
   if (!f->fn) return fail(MPKX_EUNSUPPORTED, "fused_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double) * (P.ndict > 0 ? P.ndict : 1);  // + the zero record
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double) * nd_planes(P.ndict);  // + the zero record
   f->slots = P.region_slots;
   f->matrix_bytes = (long)P.ent.size() * 4 + (long)P.vals.size() * 8;
   if (P.vals.empty()) P.vals.push_back(0.0);
@@ -121,7 +122,7 @@ int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots, long *ro
   if (slots) *slots = f->slots;
   if (row_levels) *row_levels = f->plan.row_levels;
   if (matrix_bytes) *matrix_bytes = f->matrix_bytes;
-  if (codec) *codec = !f->plan.dict ? 0 : f->plan.ndict > 0 ? 1 : 2;
+  if (codec) *codec = !f->plan.dict ? 0 : f->plan.ndict >= ND_POS ? 3 : f->plan.ndict > 0 ? 1 : 2;
   if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
   return MPKX_OK;
 }

@@ -78,10 +78,11 @@ extern "C" {
 int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *col, const double *val,
                         const int *part, int npart, int nlevel, int s, int values, double *sigma_out) {
   clear_error();
-  if (!out || values < 0 || values > 4) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
+  if (!out || values < 0 || values > 5) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
   const bool dict_ok = values == MPKX_VALUES_AUTO || values == MPKX_VALUES_AUTO_PRODUCTS ||
-                       values == MPKX_VALUES_AUTO_SELECT;
-  const int dict_codec = values == MPKX_VALUES_AUTO_PRODUCTS ? 1 : values == MPKX_VALUES_AUTO_SELECT ? 2 : 0;
+                       values == MPKX_VALUES_AUTO_SELECT || values == MPKX_VALUES_AUTO_POSITIONAL;
+  const int dict_codec = values == MPKX_VALUES_AUTO_PRODUCTS ? 1 : values == MPKX_VALUES_AUTO_SELECT ? 2
+                         : values == MPKX_VALUES_AUTO_POSITIONAL ? 3 : 0;
   auto *f = new (std::nothrow) mpkx_fused32;
   if (!f) return fail(MPKX_ENOMEM, "fused32_create");
   std::unique_ptr<mpkx_fused32> guard(f);
@@ -118,11 +119,11 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
     f->fnf = pick32<float>(P.T, P.MR, P.W, P.ndict);
     if (!f->fnf) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   }
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(float) * (P.ndict > 0 ? P.ndict : 1);  // + the zero record
+  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(float) * nd_planes(P.ndict);  // + the zero record
   f->slots = P.region_slots;
   std::vector<float> vf;
   std::vector<uint16_t> vb;
-  if (f->values == 1) {
+  if (f->values == 1 || P.ndict == ND_POS) {  // value stream, or the W positional values
     vf.resize(P.vals.size());
     for (size_t i = 0; i < vf.size(); i++) vf[i] = (float)P.vals[i];
   } else if (f->values == 2) {
@@ -181,7 +182,8 @@ int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots, long
   if (slots) *slots = f->slots;
   if (row_levels) *row_levels = f->plan.row_levels;
   if (matrix_bytes) *matrix_bytes = f->matrix_bytes;
-  if (values) *values = f->values == 0 && f->plan.ndict < 0 ? 3 : f->values;
+  if (values)
+    *values = f->values != 0 ? f->values : f->plan.ndict >= ND_POS ? 4 : f->plan.ndict < 0 ? 3 : 0;
   if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
   return MPKX_OK;
 }

@@ -35,6 +35,12 @@
 //           an entry is the slot plus a 2-bit code in bits 14-15, and the
 //           value is selected from the dictionary per entry (less LDS, more
 //           VALU; chosen when product records would cost occupancy).
+//           Positional: when every row holds the same value at ELL
+//           position k (stencils with constant coefficients, rows ordered
+//           diagonal first), an entry is the bare slot and position k's
+//           value is a kernel constant; off-diagonal positions of value
+//           -1 or +1 become a plain subtract or add (the reference's
+//           product -x or x is exact).  This is the cheapest codec.
 //           Otherwise an entry is the slot and the values stream as
 //           [row][w] (FP64, FP32 or BF16).  Padding entries point to the zero slot (an
 //           extra LDS word that is always 0) with code 0 / value 0, so
@@ -88,11 +94,20 @@ __device__ __forceinline__ float mul(float a, float x) { return a * x; }
 __device__ __forceinline__ double add(double s, double p) { return __dadd_rn(s, p); }
 __device__ __forceinline__ float add(float s, float p) { return s + p; }
 
-// LDS record of one slot: the value (ND <= 0) or its ND dictionary
-// products, one per plane of `plane` words.
+// Codec numbers (template parameter ND and TilePlan::ndict): 0 value
+// stream; 1, 2, 4 product records of that many dictionary values; -2, -4
+// select codec; ND_POS* positional values.
+constexpr int ND_POS = 8;       // position k's value from a W-entry array
+constexpr int ND_POS_NEG1 = 9;  // position 0: d0; positions >= 1: -1
+constexpr int ND_POS_POS1 = 10; // position 0: d0; positions >= 1: +1
+__host__ __device__ constexpr bool nd_products(int nd) { return nd > 0 && nd < ND_POS; }
+__host__ __device__ constexpr int nd_planes(int nd) { return nd_products(nd) ? nd : 1; }
+
+// LDS record of one slot: the value, or with product records its ND
+// dictionary products, one per plane of `plane` words.
 template <typename VT, int ND>
 __device__ __forceinline__ void put_slot(VT *buf, int slot, int plane, VT v, VT d0, VT d1, VT d2, VT d3) {
-  if constexpr (ND <= 0) {
+  if constexpr (!nd_products(ND)) {
     buf[slot] = v;
   } else {
     buf[slot] = mul(d0, v);
@@ -104,11 +119,24 @@ __device__ __forceinline__ void put_slot(VT *buf, int slot, int plane, VT v, VT 
   }
 }
 
+// Element at a 32-bit byte offset from a uniform base: the load or store
+// then takes the base in scalar registers and one VGPR of offset (every
+// array here is below 4 GiB; the plan builder checks n).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T *base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
 // VT: vector / accumulation type (double or float).  MT: stored matrix
-// value type when ND == 0 (double, float or uint16_t = bf16).  ND: 0 =
-// value stream; 1, 2, 4 = product records of that many dictionary values;
-// -2, -4 = select codec with 2 or 4 dictionary values.  inv_sigma scales
-// every new power (power of two: exact); FP64 passes 1.
+// value type when ND == 0 (double, float or uint16_t = bf16); with ND ==
+// ND_POS, vals holds the W positional values.  ND: 0 = value stream; 1, 2,
+// 4 = product records of that many dictionary values; -2, -4 = select
+// codec with 2 or 4 dictionary values; ND_POS* = positional values.
+// inv_sigma scales every new power (power of two: exact); FP64 passes 1.
 template <typename VT, typename MT, int T, int MR, int W, int ND>
 __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
                                              const uint32_t *__restrict__ ent, const MT *__restrict__ vals,
@@ -117,7 +145,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
                                              int maxslots) {
   constexpr int W2 = (W + 1) / 2;
   constexpr bool SCALE = std::is_same<VT, float>::value;
-  constexpr int NP = ND > 0 ? ND : 1;  // LDS words per slot
+  constexpr int NP = nd_planes(ND);  // LDS words per slot
   extern __shared__ __align__(16) unsigned char lds_raw[];
   VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + (size_t)(maxslots + 1) * NP;
   const int tile = xcd_remap(blockIdx.x, ntiles);
@@ -135,15 +163,22 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   const int ncomp = cnt[w - 1], nown = cnt[0];
   uint32_t e[MR][W2];
   int own[MR];
+  // Row-list loads are unconditional, on indices clamped into the tile's
+  // list (a tile has at least one own row): a load under a branch whose
+  // result is used in that branch waits for it there, which would serialise
+  // the loads of a thread.
+#pragma unroll
+  for (int m = 0; m < MR; m++) own[m] = ld_at(g, 4u * (uint32_t)min(m * T + tid, nown - 1));
+  const uint32_t *ep0 = ent + eo;
 #pragma unroll
   for (int m = 0; m < MR; m++) {
     const int r = m * T + tid;
-    const uint32_t *ep = ent + eo + (size_t)r * W2;
+    const uint32_t eoff = (uint32_t)r * (4u * W2);
     if (r < ncomp) {
       if constexpr (W2 % 4 == 0) {
 #pragma unroll
         for (int q = 0; q < W2 / 4; q++) {
-          const uint4 v4 = reinterpret_cast<const uint4 *>(ep)[q];
+          const uint4 v4 = ld_at(reinterpret_cast<const uint4 *>(ep0), eoff + 16u * q);
           e[m][4 * q] = v4.x;
           e[m][4 * q + 1] = v4.y;
           e[m][4 * q + 2] = v4.z;
@@ -151,14 +186,15 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < W2; k++) e[m][k] = ep[k];
+        for (int k = 0; k < W2; k++) e[m][k] = ld_at(ep0, eoff + 4u * k);
       }
     } else {
 #pragma unroll
       for (int k = 0; k < W2; k++) e[m][k] = 0xffffffffu;
     }
-    own[m] = r < nown ? g[r] + base : -1;
   }
+#pragma unroll
+  for (int m = 0; m < MR; m++) own[m] = m * T + tid < nown ? own[m] + base : -1;
 
   // gather: GB slots per thread in flight at a time (row indices, then
   // values, then LDS stores) so each wave keeps several loads outstanding.
@@ -167,43 +203,48 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   constexpr int GB = 8;
   const int nin = cnt[w];
   const VT *src = vv + (size_t)L0 * n;
+  // Slots past the end load the last slot's row again (same addresses, no
+  // extra traffic) and store that same value to the last slot, so every
+  // load and LDS store is unconditional (a conditional store would pull its
+  // load into the branch, and the branch waits for it).
   for (int t0 = tid; t0 < nin; t0 += GB * T) {
-    int gi[GB];
+    uint32_t gi[GB];  // row numbers
     VT gv[GB];
 #pragma unroll
-    for (int u = 0; u < GB; u++) gi[u] = t0 + u * T < nin ? g[t0 + u * T] + base : -1;
+    for (int u = 0; u < GB; u++) gi[u] = (uint32_t)ld_at(g, 4u * (uint32_t)min(t0 + u * T, nin - 1));
+#pragma unroll
+    for (int u = 0; u < GB; u++) gi[u] += (uint32_t)base;
     if constexpr (std::is_same<VT, double>::value) {  // one load path: pick the source array
       const double *sp = xin ? xin : src;
 #pragma unroll
-      for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? sp[gi[u]] : 0.0;
+      for (int u = 0; u < GB; u++) gv[u] = ld_at(sp, 8u * gi[u]);
     } else if (xin) {
 #pragma unroll
-      for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? (VT)xin[gi[u]] : (VT)0;
+      for (int u = 0; u < GB; u++) gv[u] = (VT)ld_at(xin, 8u * gi[u]);
     } else {
 #pragma unroll
-      for (int u = 0; u < GB; u++) gv[u] = gi[u] >= 0 ? src[gi[u]] : (VT)0;
+      for (int u = 0; u < GB; u++) gv[u] = ld_at(src, (uint32_t)sizeof(VT) * gi[u]);
     }
-    if constexpr (ND > 0) {  // product records do not keep x in LDS: store level 0 here
+    if constexpr (nd_products(ND)) {  // product records do not keep x in LDS: store level 0 here
       if (xin) {
 #pragma unroll
         for (int u = 0; u < GB; u++)
-          if (gi[u] >= 0 && t0 + u * T < nown) vv[gi[u]] = gv[u];
+          if (t0 + u * T < nown) st_at(vv, (uint32_t)sizeof(VT) * gi[u], gv[u]);
       }
     }
 #pragma unroll
-    for (int u = 0; u < GB; u++)
-      if (gi[u] >= 0) put_slot<VT, ND>(buf0, t0 + u * T, maxslots + 1, gv[u], d0, d1, d2, d3);
+    for (int u = 0; u < GB; u++) put_slot<VT, ND>(buf0, min(t0 + u * T, nin - 1), maxslots + 1, gv[u], d0, d1, d2, d3);
   }
   if (tid == 0) {
     put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
     put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
   }
-  if constexpr (ND <= 0) {  // LDS holds x: the own rows of level 0 go out from there
+  if constexpr (!nd_products(ND)) {  // LDS holds x: the own rows of level 0 go out from there
     if (xin) {
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < MR; m++)
-        if (own[m] >= 0) vv[own[m]] = buf0[m * T + tid];
+        if (own[m] >= 0) st_at(vv, (uint32_t)sizeof(VT) * (uint32_t)own[m], buf0[m * T + tid]);
     }
   }
 
@@ -222,12 +263,13 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
       if (r < nact) {
         // The packed words are loop invariant; passing them through an empty
         // asm makes the compiler decode them here, per level, instead of
-        // hoisting MR*W decoded values and addresses into registers.
+        // hoisting MR*W decoded values and addresses into registers.  The
+        // asm rewrites the words in place, so no copies are made.
         uint32_t ew[W2];
 #pragma unroll
         for (int k = 0; k < W2; k++) {
+          asm volatile("" : "+v"(e[m][k]));
           ew[k] = e[m][k];
-          asm volatile("" : "+v"(ew[k]));
         }
         VT s = 0;
 #pragma unroll
@@ -236,8 +278,19 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
           const uint32_t idx = (k & 1) ? wv >> 16 : wv & 0xffffu;
           if constexpr (ND == 0) {
             s = mac(s, (VT)mval_of(vp[vo + (size_t)r * W + k]), in[idx]);
-          } else if constexpr (ND > 0) {
+          } else if constexpr (nd_products(ND)) {
             s = add(s, in[idx]);
+          } else if constexpr (ND >= ND_POS) {
+            const VT xv = in[idx];
+            if constexpr (ND == ND_POS) {
+              s = mac(s, (VT)vp[k], xv);
+            } else if (k == 0) {
+              s = mac(s, d0, xv);
+            } else if constexpr (ND == ND_POS_NEG1) {
+              s = add(s, -xv);  // the product (-1) * x is exactly -x
+            } else {
+              s = add(s, xv);
+            }
           } else {
             VT a;
             if constexpr (ND == -2) {
@@ -251,7 +304,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
         }
         if (SCALE) s = s * inv_sigma;
         put_slot<VT, ND>(out, r, maxslots + 1, s, d0, d1, d2, d3);
-        if (own[m] >= 0) dst[own[m]] = s;
+        if (own[m] >= 0) st_at(dst, (uint32_t)sizeof(VT) * (uint32_t)own[m], s);
       }
       // keep the next row's LDS loads below this point: one row's operands
       // live at a time instead of MR*W of them
@@ -282,12 +335,16 @@ FusedKernel<VT, MT> pick_fused_nd(int nd) {
       case 4: return k_fused<VT, MT, T, MR, W, 4>;
       case -2: return k_fused<VT, MT, T, MR, W, -2>;
       case -4: return k_fused<VT, MT, T, MR, W, -4>;
+      case ND_POS: return k_fused<VT, MT, T, MR, W, ND_POS>;
+      case ND_POS_NEG1: return k_fused<VT, MT, T, MR, W, ND_POS_NEG1>;
+      case ND_POS_POS1: return k_fused<VT, MT, T, MR, W, ND_POS_POS1>;
     }
     return nullptr;
   }
 }
 
-// nd: 0 value stream, 1/2/4 product records, -2/-4 select codec.
+// nd: 0 value stream, 1/2/4 product records, -2/-4 select codec, ND_POS*
+// positional values.
 template <typename VT, typename MT, int T, int MR>
 FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
   switch (W) {
@@ -303,11 +360,13 @@ FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
 struct TilePlan {
   int n = 0, nlevel = 0, s = 0, ntiles = 0, W = 0, T = 0, MR = 0, maxslots = 0;
   bool dict = false;
-  int ndict = 0;              // kernel codec: 0 value stream, 1/2/4 product records, -2/-4 select
-  std::vector<double> dictv;  // up to 4 distinct values
+  int ndict = 0;              // kernel codec: 0 value stream, 1/2/4 product records, -2/-4 select,
+                              // ND_POS* positional
+  std::vector<double> dictv;  // up to 4 distinct values (positional: d0 = position 0's value)
   std::vector<int> hdr, grow;
   std::vector<uint32_t> ent;
-  std::vector<double> vals;   // value stream (explicit codec), [m][k][T] per tile
+  std::vector<double> vals;   // value stream (explicit codec), [m][k][T] per tile; ND_POS: the W
+                              // positional values
   long row_levels = 0;
   long region_slots = 0;  // sum of the tiles' region rows (before sharing)
 };
@@ -333,7 +392,9 @@ long share_segment(std::vector<V> &store, std::unordered_multimap<uint64_t, long
 
 // Builds the plan.  slot_bytes: LDS bytes per region row (2 buffers).
 // mr_choices: allowed (T, MR) pairs, smallest first.
-// dict_codec: 0 automatic, 1 product records, 2 select (dictionary plans).
+// dict_codec: 0 automatic (positional when the matrix allows it, else a
+// dictionary codec by residency), 1 product records, 2 select, 3 positional
+// (fails when the matrix does not allow it).
 inline int build_tile_plan(int n, const int *ptr, const int *col, const double *val, const int *part,
                            int npart, int nlevel, int s, bool allow_dict, int slot_bytes,
                            const std::vector<std::pair<int, int>> &mr_choices, TilePlan &P,
@@ -341,6 +402,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   if (n <= 0 || !ptr || !col || !val || !part || npart <= 0 || nlevel <= 0)
     return fail(MPKX_EINVAL, "fused: bad arguments");
   if (s < 1 || s > FUSED_MAX_S) return fail(MPKX_EINVAL, "fused: s=%d outside 1..%d", s, FUSED_MAX_S);
+  if ((long)n * (long)sizeof(double) > (long)UINT32_MAX)
+    return fail(MPKX_EUNSUPPORTED, "fused: n=%d rows exceed the 32-bit byte offsets of one power", n);
   s = std::min(s, nlevel);
   int maxdeg = 0;
   for (int i = 0; i < n; i++) maxdeg = std::max(maxdeg, ptr[i + 1] - ptr[i]);
@@ -365,6 +428,43 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   for (double d : dict)  // padding multiplies code 0 by the zero slot: keep it finite
     if (!std::isfinite(d)) use_dict = false;
   if (!use_dict) dict.clear();
+
+  // Positional values: every row holds the same (finite) value at ELL
+  // position k.  Padding positions multiply the zero slot, which adds +-0 to
+  // a sum that is never -0, so their value does not matter.
+  std::vector<double> posv(W, 0.0);
+  bool positional = allow_dict;
+  {
+    std::vector<char> seen(W, 0);
+    for (int i = 0; i < n && positional; i++)
+      for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+        const int k = j - ptr[i];
+        if (!std::isfinite(val[j])) {
+          positional = false;
+          break;
+        }
+        if (!seen[k]) {
+          posv[k] = val[j];
+          seen[k] = 1;
+        } else if (memcmp(&posv[k], &val[j], sizeof(double)) != 0) {
+          positional = false;
+          break;
+        }
+      }
+  }
+  int pos_nd = ND_POS;
+  if (positional && W > 1) {
+    bool neg = true, pos = true;
+    for (int k = 1; k < W; k++) {
+      neg = neg && (posv[k] == -1.0 || k >= maxdeg);
+      pos = pos && (posv[k] == 1.0 || k >= maxdeg);
+    }
+    pos_nd = neg ? ND_POS_NEG1 : pos ? ND_POS_POS1 : ND_POS;
+  }
+  if (dict_codec == 3 && !positional)
+    return fail(MPKX_EUNSUPPORTED, "fused: the matrix values are not constant per ELL position");
+  const bool use_pos = positional && (dict_codec == 0 || dict_codec == 3);
+  if (use_pos) use_dict = false;  // no dictionary codes in the entries
 
   std::vector<std::vector<int>> members(npart);
   for (int i = 0; i < n; i++) {
@@ -463,9 +563,9 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.T = T;
   P.MR = MR;
   P.maxslots = maxslots;
-  P.dict = use_dict;
-  P.dictv = dict;
-  P.ndict = !use_dict ? 0 : ndk == 1 ? 1 : products ? ndk : -ndk;
+  P.dict = use_dict || use_pos;  // no value stream
+  P.dictv = use_pos ? std::vector<double>{posv[0]} : dict;
+  P.ndict = use_pos ? pos_nd : !use_dict ? 0 : ndk == 1 ? 1 : products ? ndk : -ndk;
   const uint32_t plane = (uint32_t)maxslots + 1;  // LDS words per product plane
   const uint32_t zero_slot = (uint32_t)maxslots;   // the zero record (plane 0)
   P.hdr.assign((size_t)npart * FUSED_HDR, 0);
@@ -486,7 +586,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
     tmpl.resize(t.slots.size());
     for (size_t k = 0; k < t.slots.size(); k++) tmpl[k] = t.slots[k] - base;
     eseg.assign((size_t)mrt * W2 * T, 0xffffffffu);
-    vseg.assign(use_dict ? 0 : (size_t)mrt * W * T, 0.0);
+    vseg.assign(P.dict ? 0 : (size_t)mrt * W * T, 0.0);
     for (int r = 0; r < ncomp; r++) {
       for (int k = 0; k < W; k++) {
         int sl = t.rows[(size_t)r * W + k];
@@ -500,7 +600,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
         if (use_dict) {
           for (size_t c = 0; c < dict.size(); c++)
             if (memcmp(&dict[c], &v, sizeof(double)) == 0) code = (uint32_t)c;
-        } else {
+        } else if (!use_pos) {
           vseg[(size_t)r * W + k] = v;
         }
         // product records: LDS index of d_code * x; select: slot | code << 14
@@ -511,7 +611,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
     }
     long slot_off = share_segment(P.grow, gidx, tmpl);
     long ent_off = share_segment(P.ent, eidx, eseg);
-    long val_off = use_dict ? 0 : share_segment(P.vals, vidx, vseg);
+    long val_off = P.dict ? 0 : share_segment(P.vals, vidx, vseg);
     if (ent_off + (long)eseg.size() > INT32_MAX || slot_off + (long)tmpl.size() > INT32_MAX ||
         val_off + (long)vseg.size() > INT32_MAX)
       return fail(MPKX_EUNSUPPORTED, "fused: plan exceeds 32-bit offsets");
@@ -531,6 +631,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
       for (int j = 1; j <= w; j++) rl += t.cnt[w - j];
   }
   P.row_levels = rl;
+  if (P.ndict == ND_POS) P.vals = posv;
   return MPKX_OK;
 }
 

@@ -47,7 +47,7 @@ def time_fused(F, xd, vv, reps, warmup=2):
     return e0.elapsed_time(e1) / reps
 
 
-CODECS = {"auto": (0, 0), "products": (2, 3), "select": (4, 4)}  # name -> (FP64 flags, FP32 values)
+CODECS = {"auto": (0, 0), "products": (2, 3), "select": (4, 4), "positional": (8, 5)}  # name -> (FP64 flags, FP32 values)
 
 
 def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, reps=10, x=None, log=None,

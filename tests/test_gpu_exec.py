@@ -77,9 +77,8 @@ def test_fused_dict_bitwise_oracle(s):
     F = M.FusedMPK(rp, ci, va, part, k, s)
     st = F.stats()
     assert st["codec"] == "dict"
-    # 8^3 tiles: at s = 1 the product records keep 4 workgroups per CU resident,
-    # beyond that they would cost residency and the select codec is used
-    assert st["dict_codec"] == ("products" if s == 1 else "select"), st
+    # diag +6 first, every off-diagonal -1: one value per ELL position
+    assert st["dict_codec"] == "positional", st
     for graph in (False, True):
         vv = zeros_dev(n * (k + 1))
         F.exec(to_dev(x), vv, use_graph=graph)
@@ -127,22 +126,51 @@ def test_fused_shared_segments_bitwise_oracle(blocks, s):
     assert np.array_equal(to_host(vv), gold)
 
 
-@pytest.mark.parametrize("flags", [M.FusedMPK.PRODUCTS, M.FusedMPK.SELECT])
-def test_fused_forced_codec_bitwise_oracle(flags):
-    """Both dictionary codecs give the reference's bits: product records are
-    the same roundings of a_ij * x_j, formed once per slot."""
-    say("fused forced codec", flags)
-    rp, ci, va = M.gen2("cube", 40, -1.0)
+@pytest.mark.parametrize("flags", [M.FusedMPK.PRODUCTS, M.FusedMPK.SELECT, M.FusedMPK.POSITIONAL])
+@pytest.mark.parametrize("scale", [-1.0, 1.0])
+def test_fused_forced_codec_bitwise_oracle(flags, scale):
+    """Every dictionary codec gives the reference's bits: product records are
+    the same roundings of a_ij * x_j, formed once per slot; positional values
+    with off-diagonals -1 (scale -1) or +1 (scale +1, gen2's own signs)
+    subtract or add x, which is the exact product."""
+    say("fused forced codec", flags, scale)
+    rp, ci, va = M.gen2("cube", 40, scale)
     n = len(rp) - 1
     part = M.partition_grid([40, 40, 40], [4, 4, 4])
     k = 7
     x = M.uniform_pm1(9, n)
     gold = O.spmv_k(rp, ci, va, x, k)
     F = M.FusedMPK(rp, ci, va, part, k, 3, flags)
-    assert F.stats()["dict_codec"] == {M.FusedMPK.PRODUCTS: "products", M.FusedMPK.SELECT: "select"}[flags]
+    assert F.stats()["dict_codec"] == {M.FusedMPK.PRODUCTS: "products", M.FusedMPK.SELECT: "select",
+                                       M.FusedMPK.POSITIONAL: "positional"}[flags]
     vv = zeros_dev(n * (k + 1))
     F.exec(to_dev(x), vv)
     assert np.array_equal(to_host(vv), gold)
+
+
+@pytest.mark.parametrize("s", [1, 3])
+def test_fused_positional_general_values_bitwise_oracle(s):
+    """Positional values that are not +-1: entry j of every row holds c[j]
+    (a matrix built for the codec, not symmetric), so each entry multiplies by
+    its position's value; padding positions of short boundary rows add +0."""
+    say("fused positional general", s)
+    rp, ci, va = M.gen2("cube", 32, -1.0)
+    n = len(rp) - 1
+    c = np.array([6.5, -0.75, -1.25, 0.3, -1.0 / 3.0, 2.0 ** -3, -0.9])
+    pos = np.arange(len(ci)) - np.repeat(rp[:-1], np.diff(rp))
+    va = c[pos]
+    part = M.partition_grid([32, 32, 32], [4, 4, 4])
+    k = 6
+    x = M.uniform_pm1(21, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    F = M.FusedMPK(rp, ci, va, part, k, s)
+    assert F.stats()["dict_codec"] == "positional"
+    vv = zeros_dev(n * (k + 1))
+    F.exec(to_dev(x), vv)
+    assert np.array_equal(to_host(vv), gold)
+    # the same matrix with the positional codec refused: explicit values, same bits
+    with pytest.raises(M.MpkxError):
+        M.FusedMPK(rp, ci, va * (1 + 1e-3 * (np.arange(len(va)) % 2)), part, k, s, M.FusedMPK.POSITIONAL)
 
 
 def test_fused_graph_partition_bitwise_oracle():
@@ -173,7 +201,7 @@ def test_headline_size_128_cube_bitwise_oracle():
     assert np.array_equal(to_host(vv), gold)
     part = M.partition_grid([128, 128, 128], [8, 8, 8])
     F = M.FusedMPK(rp, ci, va, part, k, 3)
-    assert F.stats()["dict_codec"] == "select"  # 16^3 tiles: product records would not fit the LDS
+    assert F.stats()["dict_codec"] == "positional"  # diag +6 first, every off-diagonal -1
     vv2 = zeros_dev(n * (k + 1))
     F.exec(to_dev(x), vv2)
     out = to_host(vv2)
@@ -225,7 +253,8 @@ def _rel_errs_scaled(out32, gold, n, k, sigma):
     return np.array(errs)
 
 
-@pytest.mark.parametrize("values,tol", [("auto", 1e-5), ("fp32", 1e-5), ("products", 1e-5), ("select", 1e-5)])
+@pytest.mark.parametrize("values,tol", [("auto", 1e-5), ("fp32", 1e-5), ("products", 1e-5), ("select", 1e-5),
+                                        ("positional", 1e-5)])
 def test_fused32_cube_per_power_error(values, tol):
     """FP32 form on the 7-pt Laplacian (values exact in FP32): error vs the FP64
     oracle stays at FP32 rounding level for every power."""
@@ -239,9 +268,12 @@ def test_fused32_cube_per_power_error(values, tol):
     # 16^3 tiles (s = 4) leave no LDS room for product records; the forced
     # codecs run on 8^3 tiles
     part = M.partition_grid([N] * 3, [8, 8, 8] if values in ("products", "select") else [4, 4, 4])
-    F = M.FusedMPK32(rp, ci, va, part, k, 4, {"auto": 0, "fp32": 1, "products": 3, "select": 4}[values])
-    if values in ("products", "select"):
+    F = M.FusedMPK32(rp, ci, va, part, k, 4,
+                     {"auto": 0, "fp32": 1, "products": 3, "select": 4, "positional": 5}[values])
+    if values in ("products", "select", "positional"):
         assert F.stats()["dict_codec"] == values
+    if values == "auto":
+        assert F.stats()["dict_codec"] == "positional"
     assert F.sigma == 16.0
     vv = torch.zeros(n * (k + 1), dtype=torch.float32, device="cuda")
     F.exec(to_dev(x), vv)
