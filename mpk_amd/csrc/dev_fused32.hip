@@ -123,7 +123,7 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
   f->slots = P.region_slots;
   std::vector<float> vf;
   std::vector<uint16_t> vb;
-  if (f->values == 1 || P.ndict == ND_POS) {  // value stream, or the W positional values
+  if (f->values == 1 || (P.ndict >= ND_POS && nd_pos_kind(P.ndict) == 0)) {  // value stream, or the W positional values
     vf.resize(P.vals.size());
     for (size_t i = 0; i < vf.size(); i++) vf[i] = (float)P.vals[i];
   } else if (f->values == 2) {

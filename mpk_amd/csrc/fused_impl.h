@@ -40,7 +40,9 @@
 //           diagonal first), an entry is the bare slot and position k's
 //           value is a kernel constant; off-diagonal positions of value
 //           -1 or +1 become a plain subtract or add (the reference's
-//           product -x or x is exact).  This is the cheapest codec.
+//           product -x or x is exact).  When every row starts with its
+//           diagonal, position 0 is not stored: the row reads its own
+//           slot.  This is the cheapest codec.
 //           Otherwise an entry is the slot and the values stream as
 //           [row][w] (FP64, FP32 or BF16).  Padding entries point to the zero slot (an
 //           extra LDS word that is always 0) with code 0 / value 0, so
@@ -100,7 +102,12 @@ __device__ __forceinline__ float add(float s, float p) { return s + p; }
 constexpr int ND_POS = 8;       // position k's value from a W-entry array
 constexpr int ND_POS_NEG1 = 9;  // position 0: d0; positions >= 1: -1
 constexpr int ND_POS_POS1 = 10; // position 0: d0; positions >= 1: +1
+constexpr int ND_POSD = 11;     // ND_POS .. ND_POS_POS1 + 3: the same with the
+                                // diagonal first in every row, not stored
 __host__ __device__ constexpr bool nd_products(int nd) { return nd > 0 && nd < ND_POS; }
+__host__ __device__ constexpr bool nd_diag(int nd) { return nd >= ND_POSD; }
+// positional kind: 0 values from the array, 1 off-diagonals -1, 2 off-diagonals +1
+__host__ __device__ constexpr int nd_pos_kind(int nd) { return (nd - ND_POS) % 3; }
 __host__ __device__ constexpr int nd_planes(int nd) { return nd_products(nd) ? nd : 1; }
 
 // LDS record of one slot: the value, or with product records its ND
@@ -143,7 +150,9 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
                                              VT d0, VT d1, VT d2, VT d3, VT inv_sigma, VT *vv,
                                              const double *__restrict__ xin, int n, int L0, int w, int ntiles,
                                              int maxslots) {
-  constexpr int W2 = (W + 1) / 2;
+  constexpr bool DIAG = nd_diag(ND);   // position 0 = the row's own slot, not stored
+  constexpr int NE = W - (DIAG ? 1 : 0);  // stored entries per row
+  constexpr int W2 = (NE + 1) / 2;
   constexpr bool SCALE = std::is_same<VT, float>::value;
   constexpr int NP = nd_planes(ND);  // LDS words per slot
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -272,21 +281,23 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
           ew[k] = e[m][k];
         }
         VT s = 0;
+        if constexpr (DIAG) s = mac(s, nd_pos_kind(ND) == 0 ? (VT)vp[0] : d0, in[r]);
 #pragma unroll
-        for (int k = 0; k < W; k++) {
-          const uint32_t wv = ew[k >> 1];
-          const uint32_t idx = (k & 1) ? wv >> 16 : wv & 0xffffu;
+        for (int q = 0; q < NE; q++) {
+          const int k = q + (DIAG ? 1 : 0);  // position in the row
+          const uint32_t wv = ew[q >> 1];
+          const uint32_t idx = (q & 1) ? wv >> 16 : wv & 0xffffu;
           if constexpr (ND == 0) {
             s = mac(s, (VT)mval_of(vp[vo + (size_t)r * W + k]), in[idx]);
           } else if constexpr (nd_products(ND)) {
             s = add(s, in[idx]);
           } else if constexpr (ND >= ND_POS) {
             const VT xv = in[idx];
-            if constexpr (ND == ND_POS) {
+            if constexpr (nd_pos_kind(ND) == 0) {
               s = mac(s, (VT)vp[k], xv);
             } else if (k == 0) {
               s = mac(s, d0, xv);
-            } else if constexpr (ND == ND_POS_NEG1) {
+            } else if constexpr (nd_pos_kind(ND) == 1) {
               s = add(s, -xv);  // the product (-1) * x is exactly -x
             } else {
               s = add(s, xv);
@@ -338,6 +349,9 @@ FusedKernel<VT, MT> pick_fused_nd(int nd) {
       case ND_POS: return k_fused<VT, MT, T, MR, W, ND_POS>;
       case ND_POS_NEG1: return k_fused<VT, MT, T, MR, W, ND_POS_NEG1>;
       case ND_POS_POS1: return k_fused<VT, MT, T, MR, W, ND_POS_POS1>;
+      case ND_POSD: return k_fused<VT, MT, T, MR, W, ND_POSD>;
+      case ND_POSD + 1: return k_fused<VT, MT, T, MR, W, ND_POSD + 1>;
+      case ND_POSD + 2: return k_fused<VT, MT, T, MR, W, ND_POSD + 2>;
     }
     return nullptr;
   }
@@ -461,6 +475,9 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
     }
     pos_nd = neg ? ND_POS_NEG1 : pos ? ND_POS_POS1 : ND_POS;
   }
+  bool diag_first = true;  // every row starts with its diagonal
+  for (int i = 0; i < n && diag_first; i++) diag_first = ptr[i + 1] > ptr[i] && col[ptr[i]] == i;
+  if (diag_first) pos_nd += ND_POSD - ND_POS;
   if (dict_codec == 3 && !positional)
     return fail(MPKX_EUNSUPPORTED, "fused: the matrix values are not constant per ELL position");
   const bool use_pos = positional && (dict_codec == 0 || dict_codec == 3);
@@ -572,7 +589,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
   P.grow.clear();
   P.ent.clear();
   P.vals.clear();
-  const int W2 = (W + 1) / 2;
+  const int skip = nd_diag(P.ndict) ? 1 : 0;  // position 0 (the diagonal) not stored
+  const int W2 = (W - skip + 1) / 2;
   std::unordered_multimap<uint64_t, long> gidx, eidx, vidx;
   std::vector<int> tmpl;
   std::vector<uint32_t> eseg;
@@ -588,11 +606,14 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
     eseg.assign((size_t)mrt * W2 * T, 0xffffffffu);
     vseg.assign(P.dict ? 0 : (size_t)mrt * W * T, 0.0);
     for (int r = 0; r < ncomp; r++) {
-      for (int k = 0; k < W; k++) {
+      if (skip && t.rows[(size_t)r * W] != r)
+        return fail(MPKX_EINVAL, "fused: internal: diagonal of slot %d is not its own slot", r);
+      for (int k = skip; k < W; k++) {
         int sl = t.rows[(size_t)r * W + k];
+        const int q = k - skip;  // stored entry
         if (sl < 0) {  // padding: zero slot, code 0, value 0
-          uint32_t &word = eseg[(size_t)r * W2 + k / 2];
-          word = (k & 1) ? ((word & 0x0000ffffu) | (zero_slot << 16)) : ((word & 0xffff0000u) | zero_slot);
+          uint32_t &word = eseg[(size_t)r * W2 + q / 2];
+          word = (q & 1) ? ((word & 0x0000ffffu) | (zero_slot << 16)) : ((word & 0xffff0000u) | zero_slot);
           continue;
         }
         uint32_t code = 0;
@@ -605,8 +626,8 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
         }
         // product records: LDS index of d_code * x; select: slot | code << 14
         uint32_t e16 = P.ndict < 0 ? (uint32_t)sl | (code << 14) : code * plane + (uint32_t)sl;
-        uint32_t &word = eseg[(size_t)r * W2 + k / 2];
-        word = (k & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
+        uint32_t &word = eseg[(size_t)r * W2 + q / 2];
+        word = (q & 1) ? ((word & 0x0000ffffu) | (e16 << 16)) : ((word & 0xffff0000u) | e16);
       }
     }
     long slot_off = share_segment(P.grow, gidx, tmpl);
@@ -631,7 +652,7 @@ inline int build_tile_plan(int n, const int *ptr, const int *col, const double *
       for (int j = 1; j <= w; j++) rl += t.cnt[w - j];
   }
   P.row_levels = rl;
-  if (P.ndict == ND_POS) P.vals = posv;
+  if (P.ndict >= ND_POS && nd_pos_kind(P.ndict) == 0) P.vals = posv;
   return MPKX_OK;
 }
 

@@ -148,6 +148,54 @@ def test_fused_forced_codec_bitwise_oracle(flags, scale):
     assert np.array_equal(to_host(vv), gold)
 
 
+def _swap_first_two(rp, va):
+    """The same matrix with entries 0 and 1 of every row swapped (every row
+    of a stencil has at least 2 entries): rows no longer start with their
+    diagonal, and the CSR summation order changes with them."""
+    va = va.copy()
+    a = rp[:-1]
+    assert np.all(np.diff(rp) >= 2)
+    va[a], va[a + 1] = va[a + 1].copy(), va[a].copy()
+    return va
+
+
+def _swap_cols(rp, ci):
+    ci = ci.copy()
+    a = rp[:-1]
+    ci[a], ci[a + 1] = ci[a + 1].copy(), ci[a].copy()
+    return ci
+
+
+@pytest.mark.parametrize("values", ["general", "minus1", "plus1"])
+@pytest.mark.parametrize("diag_first", [True, False])
+def test_fused_positional_variants_bitwise_oracle(values, diag_first):
+    """Every positional kernel variant: rows with the diagonal first (the
+    diagonal is not stored) or not, and off-diagonal positions of general
+    value, all -1 or all +1.  Without the diagonal first, position 0 holds
+    the first neighbour; the summation follows the stored order either way."""
+    say("fused positional", values, diag_first)
+    N = 32
+    rp, ci, va = M.gen2("cube", N, 1.0 if values == "plus1" else -1.0)
+    n = len(rp) - 1
+    if values == "general":
+        c = np.array([6.5, -0.75, -1.25, 0.3, -1.0 / 3.0, 2.0 ** -3, -0.9])
+        va = c[np.arange(len(ci)) - np.repeat(rp[:-1], np.diff(rp))]
+    elif not diag_first:  # position 0 becomes a neighbour: make every value the unit one
+        va = np.full(len(va), -1.0 if values == "minus1" else 1.0)
+    if not diag_first:
+        ci, va = _swap_cols(rp, ci), _swap_first_two(rp, va)
+    part = M.partition_grid([N] * 3, [4, 4, 4])
+    k = 6
+    x = M.uniform_pm1(23, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    for s in (1, 3):
+        F = M.FusedMPK(rp, ci, va, part, k, s)
+        assert F.stats()["dict_codec"] == "positional"
+        vv = zeros_dev(n * (k + 1))
+        F.exec(to_dev(x), vv)
+        assert np.array_equal(to_host(vv), gold), (values, diag_first, s)
+
+
 @pytest.mark.parametrize("s", [1, 3])
 def test_fused_positional_general_values_bitwise_oracle(s):
     """Positional values that are not +-1: entry j of every row holds c[j]
