@@ -260,6 +260,13 @@ enum {
   MPKX_FUSED_SELECT = 4,   /* dictionary: per-entry select                */
   MPKX_FUSED_POSITIONAL = 8, /* one value per ELL position (fails when the */
                              /* matrix values are not constant per position) */
+  /* workgroup size (default: the smallest of 512 and 1024 threads whose */
+  /* register rows hold the tile's compute rows); also accepted by       */
+  /* mpkx_fused32_create OR-ed into values                               */
+  MPKX_FUSED_BLOCK256 = 32,
+  MPKX_FUSED_BLOCK512 = 64,
+  MPKX_FUSED_BLOCK1024 = 96,
+  MPKX_FUSED_BLOCK_MASK = 96,
 };
 /* part[n] labels in [0, npart); s = levels per launch (1..8). */
 int mpkx_fused_create(mpkx_fused **f, int n, const int *ptr, const int *col,
@@ -281,6 +288,10 @@ int mpkx_fused_round(mpkx_fused *f, double *vv, int round, void *stream);
 int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots,
                      long *row_levels, long *matrix_bytes, int *codec,
                      int *launches);
+/* Launch shape of one round: workgroups (one per tile), threads per
+ * workgroup, and compute rows held per thread. */
+int mpkx_fused_launch_info(const mpkx_fused *f, int *grid, int *block,
+                           int *rows_per_thread);
 
 /* ------------------------------------------------------------------ */
 /* pdmpk "partial diamond" schedule (pdmpk/comm_comp_patterns.cc):     */
@@ -353,6 +364,8 @@ int mpkx_fused32_round(mpkx_fused32 *f, float *vv, int round, void *stream);
 int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots,
                        long *row_levels, long *matrix_bytes, int *values,
                        int *launches);
+int mpkx_fused32_launch_info(const mpkx_fused32 *f, int *grid, int *block,
+                             int *rows_per_thread);
 /* FP64 checks on the GPU.  norms[l] = ||vv[l*n ..]||_2 for l = 0..nlevel
  * (host array of nlevel+1).  residual = ||sigma*v_l - A v_{l-1}||_2 /
  * ||A v_{l-1}||_2 with A applied in FP64 (the mpkx_csr matrix). */

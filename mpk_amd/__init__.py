@@ -115,6 +115,8 @@ def lib():
         "mpkx_fused_destroy": ([P], None),
         "mpkx_fused_exec": ([P, P, P, P, C.c_int], C.c_int),
         "mpkx_fused_stats": ([P, P, P, P, P, P, P], C.c_int),
+        "mpkx_fused_launch_info": ([P, P, P, P], C.c_int),
+        "mpkx_fused32_launch_info": ([P, P, P, P], C.c_int),
         "mpkx_fused_round": ([P, P, C.c_int, P], C.c_int),
         "mpkx_fused32_create": ([P, C.c_int, P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, P], C.c_int),
         "mpkx_fused32_destroy": ([P], None),
@@ -588,10 +590,17 @@ class CsrMatrix:
             self._h = None
 
 
+def _launch_info(fn, h):
+    g, b, p = C.c_int(), C.c_int(), C.c_int()
+    _check(fn(h, C.byref(g), C.byref(b), C.byref(p)))
+    return {"grid": g.value, "block": b.value, "rows_per_thread": p.value}
+
+
 class FusedMPK:
     """Fused skirt-round MPK: s powers per launch, tiles from `part`."""
 
     EXPLICIT, PRODUCTS, SELECT, POSITIONAL = 1, 2, 4, 8
+    BLOCK256, BLOCK512, BLOCK1024 = 32, 64, 96  # workgroup size (also OR-ed into FusedMPK32 values)
 
     def __init__(self, ptr, col, val, part, nlevel, s, flags=0):
         self._h = C.c_void_p()
@@ -619,7 +628,7 @@ class FusedMPK:
         return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
                 "matrix_bytes": mb.value, "codec": ["explicit", "dict", "dict", "dict"][codec.value],
                 "dict_codec": [None, "products", "select", "positional"][codec.value],
-                "launches": la.value}
+                "launches": la.value, **_launch_info(lib().mpkx_fused_launch_info, self._h)}
 
     def __del__(self, _release=_release):
         if getattr(self, "_h", None):
@@ -676,7 +685,7 @@ class FusedMPK32:
         return {"max_slots": ms.value, "slots": sl.value, "row_levels": rl.value,
                 "matrix_bytes": mb.value, "values": ["dict", "fp32", "bf16", "dict", "dict"][vals.value],
                 "dict_codec": {0: "products", 3: "select", 4: "positional"}.get(vals.value),
-                "launches": la.value, "sigma": self.sigma}
+                "launches": la.value, "sigma": self.sigma, **_launch_info(lib().mpkx_fused32_launch_info, self._h)}
 
     def __del__(self, _release=_release):
         if getattr(self, "_h", None):

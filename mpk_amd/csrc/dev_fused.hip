@@ -12,6 +12,8 @@ namespace {
 using K64 = FusedKernel<double, double>;
 
 K64 pick64(int T, int MR, int W, int nd) {
+  if (T == 256 && MR == 4) return pick_fused_w<double, double, 256, 4>(W, nd);
+  if (T == 256 && MR == 8) return pick_fused_w<double, double, 256, 8>(W, nd);
   if (T == 512 && MR == 4) return pick_fused_w<double, double, 512, 4>(W, nd);
   if (T == 1024 && MR == 4) return pick_fused_w<double, double, 1024, 4>(W, nd);
   if (T == 1024 && MR == 8) return pick_fused_w<double, double, 1024, 8>(W, nd);
@@ -57,8 +59,9 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   std::unique_ptr<mpkx_fused> guard(f);
   int rc;
   try {
+    std::vector<std::pair<int, int>> mr = block_choices(flags & MPKX_FUSED_BLOCK_MASK, {{512, 4}, {1024, 4}, {1024, 8}});
     rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, !(flags & MPKX_FUSED_EXPLICIT),
-                         2 * (int)sizeof(double), {{512, 4}, {1024, 4}, {1024, 8}}, f->plan,
+                         2 * (int)sizeof(double), mr, f->plan,
                          (flags & MPKX_FUSED_PRODUCTS) ? 1 : (flags & MPKX_FUSED_SELECT) ? 2
                          : (flags & MPKX_FUSED_POSITIONAL) ? 3 : 0);
   } catch (const std::bad_alloc &) {
@@ -124,6 +127,15 @@ int mpkx_fused_stats(const mpkx_fused *f, long *max_slots, long *slots, long *ro
   if (matrix_bytes) *matrix_bytes = f->matrix_bytes;
   if (codec) *codec = !f->plan.dict ? 0 : f->plan.ndict >= ND_POS ? 3 : f->plan.ndict > 0 ? 1 : 2;
   if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
+  return MPKX_OK;
+}
+
+int mpkx_fused_launch_info(const mpkx_fused *f, int *grid, int *block, int *rows_per_thread) {
+  clear_error();
+  if (!f) return fail(MPKX_EINVAL, "fused_launch_info: null");
+  if (grid) *grid = f->plan.ntiles;
+  if (block) *block = f->plan.T;
+  if (rows_per_thread) *rows_per_thread = f->plan.MR;
   return MPKX_OK;
 }
 

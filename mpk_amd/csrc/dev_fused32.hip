@@ -19,6 +19,8 @@ using K32b = FusedKernel<float, uint16_t>;
 
 template <typename MT>
 FusedKernel<float, MT> pick32(int T, int MR, int W, int nd) {
+  if (T == 256 && MR == 4) return pick_fused_w<float, MT, 256, 4>(W, nd);
+  if (T == 256 && MR == 8) return pick_fused_w<float, MT, 256, 8>(W, nd);
   if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4>(W, nd);
   if (T == 1024 && MR == 4) return pick_fused_w<float, MT, 1024, 4>(W, nd);
   if (T == 1024 && MR == 8) return pick_fused_w<float, MT, 1024, 8>(W, nd);
@@ -78,6 +80,8 @@ extern "C" {
 int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *col, const double *val,
                         const int *part, int npart, int nlevel, int s, int values, double *sigma_out) {
   clear_error();
+  const int block = values & MPKX_FUSED_BLOCK_MASK;
+  values &= ~MPKX_FUSED_BLOCK_MASK;
   if (!out || values < 0 || values > 5) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
   const bool dict_ok = values == MPKX_VALUES_AUTO || values == MPKX_VALUES_AUTO_PRODUCTS ||
                        values == MPKX_VALUES_AUTO_SELECT || values == MPKX_VALUES_AUTO_POSITIONAL;
@@ -89,7 +93,8 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
   int rc;
   try {
     rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, dict_ok, 2 * (int)sizeof(float),
-                         {{512, 4}, {1024, 4}, {1024, 8}, {1024, 12}}, f->plan, dict_codec);
+                         block_choices(block, {{512, 4}, {1024, 4}, {1024, 8}, {1024, 12}}), f->plan,
+                         dict_codec);
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fused32_create: host allocation");
   }
@@ -185,6 +190,15 @@ int mpkx_fused32_stats(const mpkx_fused32 *f, long *max_slots, long *slots, long
   if (values)
     *values = f->values != 0 ? f->values : f->plan.ndict >= ND_POS ? 4 : f->plan.ndict < 0 ? 3 : 0;
   if (launches) *launches = (f->plan.nlevel + f->plan.s - 1) / f->plan.s;
+  return MPKX_OK;
+}
+
+int mpkx_fused32_launch_info(const mpkx_fused32 *f, int *grid, int *block, int *rows_per_thread) {
+  clear_error();
+  if (!f) return fail(MPKX_EINVAL, "fused32_launch_info: null");
+  if (grid) *grid = f->plan.ntiles;
+  if (block) *block = f->plan.T;
+  if (rows_per_thread) *rows_per_thread = f->plan.MR;
   return MPKX_OK;
 }
 

@@ -138,12 +138,101 @@ __device__ __forceinline__ void st_at(T *base, uint32_t byte_off, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
 }
 
+// One tile's plan data (uniform across the workgroup).
+struct TileRef {
+  const int *g;         // region rows, relative to base, sorted by (distance, row)
+  const uint32_t *ep;   // ELL block
+  const int *cnt;       // cnt[d]: region rows at distance <= d
+  uint32_t vo;          // value stream offset (explicit codec)
+  int base, nin, ncomp, nown;
+};
+
+__device__ __forceinline__ TileRef tile_ref(const int *hdr, const int *grow, const uint32_t *ent, int tile, int w) {
+  const int *h = hdr + (size_t)tile * FUSED_HDR;
+  TileRef t;
+  t.g = grow + h[0];
+  t.ep = ent + (size_t)(unsigned)h[1];
+  t.cnt = h + 4;
+  t.vo = (uint32_t)h[2];
+  t.base = h[FUSED_HDR - 1];
+  t.nin = h[4 + w];
+  t.ncomp = h[4 + w - 1];
+  t.nown = h[4];
+  return t;
+}
+
+// Row numbers of region slots t0 + u*T.  Slots past the end load the last
+// slot's row again (same addresses, no extra traffic) and later store that
+// same value to the last slot, so every load and LDS store is
+// unconditional: a load under a branch whose result is used in that branch
+// waits for it there, which would serialise the loads of a thread.
+template <int GB, int T>
+__device__ __forceinline__ void gather_rows(const TileRef &t, int t0, uint32_t (&gi)[GB]) {
+#pragma unroll
+  for (int u = 0; u < GB; u++) gi[u] = (uint32_t)ld_at(t.g, 4u * (uint32_t)min(t0 + u * T, t.nin - 1));
+#pragma unroll
+  for (int u = 0; u < GB; u++) gi[u] += (uint32_t)t.base;
+}
+
+// Level-L0 values of those rows: from x (FP64) in the first round of an
+// exec, else from the Krylov block.
+template <typename VT, int GB>
+__device__ __forceinline__ void gather_vals(const uint32_t (&gi)[GB], const double *xin, const VT *src,
+                                            VT (&gv)[GB]) {
+  if constexpr (std::is_same<VT, double>::value) {  // one load path: pick the source array
+    const double *sp = xin ? xin : src;
+#pragma unroll
+    for (int u = 0; u < GB; u++) gv[u] = ld_at(sp, 8u * gi[u]);
+  } else if (xin) {
+#pragma unroll
+    for (int u = 0; u < GB; u++) gv[u] = (VT)ld_at(xin, 8u * gi[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < GB; u++) gv[u] = ld_at(src, (uint32_t)sizeof(VT) * gi[u]);
+  }
+}
+
+// Matrix rows of this thread into registers (row r = m*T + tid; W2
+// consecutive words per row, one wide load when W2 % 4 == 0) and the HBM
+// row numbers of its own rows (-1: not an own row).
+template <int T, int MR, int W2>
+__device__ __forceinline__ void load_rows(const TileRef &t, int tid, uint32_t (&e)[MR][W2], int (&own)[MR]) {
+#pragma unroll
+  for (int m = 0; m < MR; m++) own[m] = ld_at(t.g, 4u * (uint32_t)min(m * T + tid, t.nown - 1));
+#pragma unroll
+  for (int m = 0; m < MR; m++) {
+    const int r = m * T + tid;
+    const uint32_t eoff = (uint32_t)r * (4u * W2);
+    if (r < t.ncomp) {
+      if constexpr (W2 % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < W2 / 4; q++) {
+          const uint4 v4 = ld_at(reinterpret_cast<const uint4 *>(t.ep), eoff + 16u * q);
+          e[m][4 * q] = v4.x;
+          e[m][4 * q + 1] = v4.y;
+          e[m][4 * q + 2] = v4.z;
+          e[m][4 * q + 3] = v4.w;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < W2; k++) e[m][k] = ld_at(t.ep, eoff + 4u * k);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < W2; k++) e[m][k] = 0xffffffffu;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MR; m++) own[m] = m * T + tid < t.nown ? own[m] + t.base : -1;
+}
+
 // VT: vector / accumulation type (double or float).  MT: stored matrix
-// value type when ND == 0 (double, float or uint16_t = bf16); with ND ==
-// ND_POS, vals holds the W positional values.  ND: 0 = value stream; 1, 2,
-// 4 = product records of that many dictionary values; -2, -4 = select
-// codec with 2 or 4 dictionary values; ND_POS* = positional values.
-// inv_sigma scales every new power (power of two: exact); FP64 passes 1.
+// value type when ND == 0 (double, float or uint16_t = bf16); with
+// positional values of kind 0, vals holds the W positional values.  ND: 0 =
+// value stream; 1, 2, 4 = product records of that many dictionary values;
+// -2, -4 = select codec with 2 or 4 dictionary values; ND_POS* =
+// positional values.  inv_sigma scales every new power (power of two:
+// exact); FP64 passes 1.  One workgroup per tile.
 template <typename VT, typename MT, int T, int MR, int W, int ND>
 __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
                                              const uint32_t *__restrict__ ent, const MT *__restrict__ vals,
@@ -155,95 +244,40 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   constexpr int W2 = (NE + 1) / 2;
   constexpr bool SCALE = std::is_same<VT, float>::value;
   constexpr int NP = nd_planes(ND);  // LDS words per slot
+  constexpr int GB = 8;              // gather batch: slots per thread in flight at a time
   extern __shared__ __align__(16) unsigned char lds_raw[];
   VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + (size_t)(maxslots + 1) * NP;
-  const int tile = xcd_remap(blockIdx.x, ntiles);
-  const int *h = hdr + (size_t)tile * FUSED_HDR;
-  const int *g = grow + h[0];
-  const size_t eo = (size_t)(unsigned)h[1];
-  const size_t vo = (size_t)(unsigned)h[2];
-  const int *cnt = h + 4;
-  const int base = h[FUSED_HDR - 1];
   const int tid = threadIdx.x;
+  const VT *src = vv + (size_t)L0 * n;
+  const TileRef t = tile_ref(hdr, grow, ent, xcd_remap(blockIdx.x, ntiles), w);
 
-  // matrix rows of this thread into registers (W2 consecutive words per
-  // row: one wide load) and own-row targets, issued first so that their
-  // latency overlaps the gather
-  const int ncomp = cnt[w - 1], nown = cnt[0];
+  // matrix rows of this thread and own-row targets, issued first so that
+  // their latency overlaps the gather
   uint32_t e[MR][W2];
   int own[MR];
-  // Row-list loads are unconditional, on indices clamped into the tile's
-  // list (a tile has at least one own row): a load under a branch whose
-  // result is used in that branch waits for it there, which would serialise
-  // the loads of a thread.
-#pragma unroll
-  for (int m = 0; m < MR; m++) own[m] = ld_at(g, 4u * (uint32_t)min(m * T + tid, nown - 1));
-  const uint32_t *ep0 = ent + eo;
-#pragma unroll
-  for (int m = 0; m < MR; m++) {
-    const int r = m * T + tid;
-    const uint32_t eoff = (uint32_t)r * (4u * W2);
-    if (r < ncomp) {
-      if constexpr (W2 % 4 == 0) {
-#pragma unroll
-        for (int q = 0; q < W2 / 4; q++) {
-          const uint4 v4 = ld_at(reinterpret_cast<const uint4 *>(ep0), eoff + 16u * q);
-          e[m][4 * q] = v4.x;
-          e[m][4 * q + 1] = v4.y;
-          e[m][4 * q + 2] = v4.z;
-          e[m][4 * q + 3] = v4.w;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < W2; k++) e[m][k] = ld_at(ep0, eoff + 4u * k);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < W2; k++) e[m][k] = 0xffffffffu;
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < MR; m++) own[m] = m * T + tid < nown ? own[m] + base : -1;
+  load_rows<T, MR, W2>(t, tid, e, own);
 
-  // gather: GB slots per thread in flight at a time (row indices, then
-  // values, then LDS stores) so each wave keeps several loads outstanding.
-  // With xin (the first round of an exec) level 0 is read from x itself and
-  // the tile writes its own rows of level 0 into vv, replacing a copy pass.
-  constexpr int GB = 8;
-  const int nin = cnt[w];
-  const VT *src = vv + (size_t)L0 * n;
-  // Slots past the end load the last slot's row again (same addresses, no
-  // extra traffic) and store that same value to the last slot, so every
-  // load and LDS store is unconditional (a conditional store would pull its
-  // load into the branch, and the branch waits for it).
-  for (int t0 = tid; t0 < nin; t0 += GB * T) {
-    uint32_t gi[GB];  // row numbers
+  // Level L0 of the region into LDS, GB slots per thread in flight at a
+  // time.  With xin (the first round of an exec) level 0 is read from x
+  // itself and the tile writes its own rows of level 0 into vv, replacing a
+  // copy pass.
+  for (int t0 = tid; t0 - tid < t.nin; t0 += GB * T) {
+    uint32_t gi[GB];
     VT gv[GB];
-#pragma unroll
-    for (int u = 0; u < GB; u++) gi[u] = (uint32_t)ld_at(g, 4u * (uint32_t)min(t0 + u * T, nin - 1));
-#pragma unroll
-    for (int u = 0; u < GB; u++) gi[u] += (uint32_t)base;
-    if constexpr (std::is_same<VT, double>::value) {  // one load path: pick the source array
-      const double *sp = xin ? xin : src;
-#pragma unroll
-      for (int u = 0; u < GB; u++) gv[u] = ld_at(sp, 8u * gi[u]);
-    } else if (xin) {
-#pragma unroll
-      for (int u = 0; u < GB; u++) gv[u] = (VT)ld_at(xin, 8u * gi[u]);
-    } else {
-#pragma unroll
-      for (int u = 0; u < GB; u++) gv[u] = ld_at(src, (uint32_t)sizeof(VT) * gi[u]);
-    }
+    gather_rows<GB, T>(t, t0, gi);
+    gather_vals<VT, GB>(gi, xin, src, gv);
     if constexpr (nd_products(ND)) {  // product records do not keep x in LDS: store level 0 here
       if (xin) {
 #pragma unroll
         for (int u = 0; u < GB; u++)
-          if (t0 + u * T < nown) st_at(vv, (uint32_t)sizeof(VT) * gi[u], gv[u]);
+          if (t0 + u * T < t.nown) st_at(vv, (uint32_t)sizeof(VT) * gi[u], gv[u]);
       }
     }
 #pragma unroll
-    for (int u = 0; u < GB; u++) put_slot<VT, ND>(buf0, min(t0 + u * T, nin - 1), maxslots + 1, gv[u], d0, d1, d2, d3);
+    for (int u = 0; u < GB; u++)
+      put_slot<VT, ND>(buf0, min(t0 + u * T, t.nin - 1), maxslots + 1, gv[u], d0, d1, d2, d3);
   }
+  // the zero record: padding entries point to it
   if (tid == 0) {
     put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
     put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
@@ -256,12 +290,11 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
         if (own[m] >= 0) st_at(vv, (uint32_t)sizeof(VT) * (uint32_t)own[m], buf0[m * T + tid]);
     }
   }
-
   __syncthreads();
 
   VT *in = buf0, *out = buf1;
   for (int j = 1; j <= w; j++) {
-    const int nact = cnt[w - j];
+    const int nact = t.cnt[w - j];
     VT *dst = vv + (size_t)(L0 + j) * n;
     // value stream re-read per level (L2 resident): keep its loads in the loop
     const MT *vp = vals;
@@ -288,7 +321,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
           const uint32_t wv = ew[q >> 1];
           const uint32_t idx = (q & 1) ? wv >> 16 : wv & 0xffffu;
           if constexpr (ND == 0) {
-            s = mac(s, (VT)mval_of(vp[vo + (size_t)r * W + k]), in[idx]);
+            s = mac(s, (VT)mval_of(vp[t.vo + (size_t)r * W + k]), in[idx]);
           } else if constexpr (nd_products(ND)) {
             s = add(s, in[idx]);
           } else if constexpr (ND >= ND_POS) {
@@ -322,9 +355,9 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
       __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
-    VT *t = in;
+    VT *tmp = in;
     in = out;
-    out = t;
+    out = tmp;
   }
 }
 
@@ -368,6 +401,19 @@ FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
     case 16: return pick_fused_nd<VT, MT, T, MR, 16>(nd);
   }
   return nullptr;
+}
+
+// (threads per workgroup, register rows per thread) pairs the plan builder
+// may pick from, smallest first: MPKX_FUSED_BLOCK256 / 512 / 1024 restrict
+// them to one workgroup size (256 adds 256x4 and 256x8), else dflt.
+inline std::vector<std::pair<int, int>> block_choices(int block_flags,
+                                                      const std::vector<std::pair<int, int>> &dflt) {
+  if (block_flags == MPKX_FUSED_BLOCK256) return {{256, 4}, {256, 8}};
+  std::vector<std::pair<int, int>> out;
+  const int want = block_flags == MPKX_FUSED_BLOCK512 ? 512 : block_flags == MPKX_FUSED_BLOCK1024 ? 1024 : 0;
+  for (auto &c : dflt)
+    if (!want || c.first == want) out.push_back(c);
+  return out;
 }
 
 // Host-side tile plan.

@@ -47,7 +47,10 @@ def time_fused(F, xd, vv, reps, warmup=2):
     return e0.elapsed_time(e1) / reps
 
 
-CODECS = {"auto": (0, 0), "products": (2, 3), "select": (4, 4), "positional": (8, 5)}  # name -> (FP64 flags, FP32 values)
+# name -> (FP64 flags, FP32 values); "+b256" etc.: that workgroup size
+CODECS = {"auto": (0, 0), "products": (2, 3), "select": (4, 4), "positional": (8, 5)}
+for _b, _f in (("256", 32), ("512", 64), ("1024", 96)):
+    CODECS.update({c + "+b" + _b: (f | _f, v | _f) for c, (f, v) in list(CODECS.items()) if "+" not in c})
 
 
 def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, reps=10, x=None, log=None,

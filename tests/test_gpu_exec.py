@@ -196,6 +196,55 @@ def test_fused_positional_variants_bitwise_oracle(values, diag_first):
         assert np.array_equal(to_host(vv), gold), (values, diag_first, s)
 
 
+@pytest.mark.parametrize("codec", ["auto", "select", "explicit"])
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_fused_block_sizes_bitwise_oracle(codec, block):
+    """Every workgroup size (threads per tile, register rows per thread)
+    gives the reference's bits, for every s."""
+    say("fused block", codec, block)
+    N, k = 48, 7
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    if codec == "explicit":
+        va = va * (1 + 0.1 * np.random.default_rng(13).standard_normal(len(va)))
+    n = len(rp) - 1
+    part = M.partition_grid([N] * 3, [6, 6, 6])
+    x = M.uniform_pm1(17, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    flags = {256: M.FusedMPK.BLOCK256, 512: M.FusedMPK.BLOCK512, 1024: M.FusedMPK.BLOCK1024}[block]
+    flags |= {"auto": 0, "select": M.FusedMPK.SELECT, "explicit": 0}[codec]
+    for s in (1, 2, 3):
+        try:
+            F = M.FusedMPK(rp, ci, va, part, k, s, flags)
+        except M.MpkxError as e:  # too many compute rows for 256 threads x 8 rows
+            assert block == 256 and "register blocking" in str(e)
+            continue
+        st = F.stats()
+        assert st["block"] == block and st["grid"] == 216, st
+        assert st["dict_codec"] == {"auto": "positional", "select": "select", "explicit": None}[codec], st
+        vv = zeros_dev(n * (k + 1))
+        F.exec(to_dev(x), vv)
+        assert np.array_equal(to_host(vv), gold), (codec, block, s)
+
+
+def test_fused32_block_sizes_agree():
+    """FP32 rounds give the same bits for every workgroup size (the order of
+    operations per row does not depend on it)."""
+    import torch
+    say("fused32 block sizes")
+    N, k = 64, 8
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    n = len(rp) - 1
+    part = M.partition_grid([N] * 3, [8, 8, 8])
+    x = to_dev(M.uniform_pm1(0x5EED, n))
+    outs = []
+    for block in (M.FusedMPK.BLOCK256, M.FusedMPK.BLOCK512, M.FusedMPK.BLOCK1024):
+        F = M.FusedMPK32(rp, ci, va, part, k, 2, M.FusedMPK32.AUTO | block)
+        vv = torch.zeros(n * (k + 1), dtype=torch.float32, device="cuda")
+        F.exec(x, vv)
+        outs.append(to_host(vv))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
 @pytest.mark.parametrize("s", [1, 3])
 def test_fused_positional_general_values_bitwise_oracle(s):
     """Positional values that are not +-1: entry j of every row holds c[j]
@@ -256,6 +305,7 @@ def test_headline_size_128_cube_bitwise_oracle():
     err = per_power_rel_err(out, gold, n, k)
     say("per-power normwise error", err)
     assert np.array_equal(out, gold)
+    assert F.stats()["max_slots"] > 8 * F.stats()["block"]  # several gather batches per tile
 
 
 def test_exact_plan_headline_size_bitwise_oracle():
