@@ -109,10 +109,10 @@ def cpu_baseline_reference(M, rp, ci, va, x, N, k, gold_check):
             "results_bitwise_equal_gpu": ok}
 
 
-def measured_traffic(N, k, s, blocks):
+def measured_traffic(N, k, s, blocks, codec):
     """HBM bytes per launch of the dominant fused round, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this configuration (None if
-    this configuration was not profiled)."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this configuration and value
+    codec (None if this configuration was not profiled)."""
     path = os.path.join(REPO, "profiles", "k_fused_traffic.json")
     try:
         with open(path) as f:
@@ -120,7 +120,8 @@ def measured_traffic(N, k, s, blocks):
     except (OSError, ValueError):
         return None
     cfg = rec.get("config", {})
-    if cfg.get("N") == N and cfg.get("k") == k and cfg.get("s") == s and cfg.get("blocks") == blocks:
+    if (cfg.get("N") == N and cfg.get("k") == k and cfg.get("s") == s and cfg.get("blocks") == blocks
+            and cfg.get("codec") == codec):
         return rec.get("hbm_bytes_per_launch")
     return None
 
@@ -556,7 +557,7 @@ def main():
     r_dom = int(np.argmax(round_ms))
     w_dom = min(s, k - r_dom * s)
     achieved = w_dom * B / (round_ms[r_dom] * 1e-3) / 1e9
-    traffic = measured_traffic(N, k, s, blocks)
+    traffic = measured_traffic(N, k, s, blocks, st["dict_codec"])
 
     out = {
         "metric": "MPK effective GB/s & % HBM roofline, fused vs kxSpMV, 3D 7-pt k=8",
