@@ -245,6 +245,11 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   constexpr bool SCALE = std::is_same<VT, float>::value;
   constexpr int NP = nd_planes(ND);  // LDS words per slot
   constexpr int GB = 8;              // gather batch: slots per thread in flight at a time
+  // Row r = m*T + tid is gathered by this thread (first batch, u = m) and
+  // computed by it at every level, so its previous-level value can stay in
+  // a register (XREG) instead of being read back from LDS.  Not with 8 or
+  // more register rows: the extra registers would cost occupancy.
+  constexpr bool XREG = MR <= 4 && MR <= GB;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + (size_t)(maxslots + 1) * NP;
   const int tid = threadIdx.x;
@@ -256,6 +261,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   uint32_t e[MR][W2];
   int own[MR];
   load_rows<T, MR, W2>(t, tid, e, own);
+  VT xr[MR];  // XREG: the value of row m*T + tid at the previous level
 
   // Level L0 of the region into LDS, GB slots per thread in flight at a
   // time.  With xin (the first round of an exec) level 0 is read from x
@@ -276,18 +282,24 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
 #pragma unroll
     for (int u = 0; u < GB; u++)
       put_slot<VT, ND>(buf0, min(t0 + u * T, t.nin - 1), maxslots + 1, gv[u], d0, d1, d2, d3);
+    if constexpr (XREG) {
+      if (t0 == tid) {  // first batch (uniform)
+#pragma unroll
+        for (int m = 0; m < MR; m++) xr[m] = gv[m];
+      }
+    }
   }
   // the zero record: padding entries point to it
   if (tid == 0) {
     put_slot<VT, ND>(buf0, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
     put_slot<VT, ND>(buf1, maxslots, maxslots + 1, (VT)0, d0, d1, d2, d3);
   }
-  if constexpr (!nd_products(ND)) {  // LDS holds x: the own rows of level 0 go out from there
+  if constexpr (!nd_products(ND)) {  // the own rows of level 0 go out from registers or LDS
     if (xin) {
-      __syncthreads();
+      if constexpr (!XREG) __syncthreads();
 #pragma unroll
       for (int m = 0; m < MR; m++)
-        if (own[m] >= 0) st_at(vv, (uint32_t)sizeof(VT) * (uint32_t)own[m], buf0[m * T + tid]);
+        if (own[m] >= 0) st_at(vv, (uint32_t)sizeof(VT) * (uint32_t)own[m], XREG ? xr[m] : buf0[m * T + tid]);
     }
   }
   __syncthreads();
@@ -314,7 +326,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
           ew[k] = e[m][k];
         }
         VT s = 0;
-        if constexpr (DIAG) s = mac(s, nd_pos_kind(ND) == 0 ? (VT)vp[0] : d0, in[r]);
+        if constexpr (DIAG) s = mac(s, nd_pos_kind(ND) == 0 ? (VT)vp[0] : d0, XREG ? xr[m] : in[r]);
 #pragma unroll
         for (int q = 0; q < NE; q++) {
           const int k = q + (DIAG ? 1 : 0);  // position in the row
@@ -347,6 +359,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
           }
         }
         if (SCALE) s = s * inv_sigma;
+        if constexpr (XREG) xr[m] = s;
         put_slot<VT, ND>(out, r, maxslots + 1, s, d0, d1, d2, d3);
         if (own[m] >= 0) st_at(dst, (uint32_t)sizeof(VT) * (uint32_t)own[m], s);
       }
