@@ -193,8 +193,18 @@ int mpkx_prep_mpk(int n, const int *ptr, const int *col, int nlevel,
 typedef struct mpkx_plan mpkx_plan;
 
 /* Uploads the programs of bufs[0..npart) (rank p at index p; every
- * bufs[p]->mval_buf must be set). */
+ * bufs[p]->mval_buf must be set).  When every row holds the same value at
+ * each position (stencils with constant coefficients) the plan keeps one
+ * value per position instead of mval_buf (same products, same sums). */
 int mpkx_plan_create(mpkx_plan **plan, mpkx_buffers *const *bufs, int npart);
+/* The same with options: MPKX_PLAN_EXPLICIT keeps one value per entry. */
+enum { MPKX_PLAN_EXPLICIT = 1 };
+int mpkx_plan_create_opts(mpkx_plan **plan, mpkx_buffers *const *bufs,
+                          int npart, int flags);
+/* positional: 1 when the plan keeps positional values; matrix_bytes:
+ * device bytes of its row pointers, columns and values. */
+int mpkx_plan_info(const mpkx_plan *plan, int *positional,
+                   long *matrix_bytes);
 void mpkx_plan_destroy(mpkx_plan *plan);
 /* Host drop-in: takes each bufs[p]->vv_buf[0 .. rcount[0]) as the input
  * (as the reference main loads it), runs every phase, and writes every

@@ -91,6 +91,8 @@ def lib():
         "mpkx_del_bufs_array": ([P, C.c_int], None),
         "mpkx_prep_mpk": ([C.c_int, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P, P, P], C.c_int),
         "mpkx_plan_create": ([P, P, C.c_int], C.c_int),
+        "mpkx_plan_create_opts": ([P, P, C.c_int, C.c_int], C.c_int),
+        "mpkx_plan_info": ([P, P, P], C.c_int),
         "mpkx_plan_destroy": ([P], None),
         "mpkx_exec_bufs": ([P, P], C.c_int),
         "mpkx_exec": ([P, P, P, P, C.c_int], C.c_int),
@@ -472,9 +474,11 @@ def prep_mpk(ptr, col, nlevel, nphase, parts, levels, skirt):
 class Plan:
     """GPU exec of reference buffers (drop-in for mpk_exec_bufs_val on P ranks)."""
 
-    def __init__(self, bufset):
+    EXPLICIT = 1  # keep one value per entry (no positional values)
+
+    def __init__(self, bufset, flags=0):
         self._h = C.c_void_p()
-        _check(lib().mpkx_plan_create(C.byref(self._h), bufset.array(), bufset.npart))
+        _check(lib().mpkx_plan_create_opts(C.byref(self._h), bufset.array(), bufset.npart, flags))
 
     def exec_bufs(self, bufset):
         _check(lib().mpkx_exec_bufs(self._h, bufset.array()))
@@ -485,7 +489,10 @@ class Plan:
     def stats(self):
         r, z, g, l = C.c_long(), C.c_long(), C.c_long(), C.c_int()
         _check(lib().mpkx_plan_stats(self._h, C.byref(r), C.byref(z), C.byref(g), C.byref(l)))
-        return {"rows": r.value, "nnz": z.value, "ghosts": g.value, "launches": l.value}
+        pos, mb = C.c_int(), C.c_long()
+        _check(lib().mpkx_plan_info(self._h, C.byref(pos), C.byref(mb)))
+        return {"rows": r.value, "nnz": z.value, "ghosts": g.value, "launches": l.value,
+                "positional": bool(pos.value), "matrix_bytes": mb.value}
 
     def __del__(self, _release=_release):
         if getattr(self, "_h", None):

@@ -15,6 +15,12 @@
 // rounded separately (no fused multiply-add), the same operation sequence
 // as the reference's `tmp += mval[mj] * vv_buf[mcol[mj]]`, so results are
 // bitwise equal to the reference and to verify_val.
+//
+// Positional values (POS): when every row of the buffers holds the same
+// value at each position (stencils with constant coefficients), the plan
+// keeps those W values instead of one value per entry; the workgroup
+// stages x[mcol] and multiplies by the position's value while summing, the
+// same product and the same sum, with 4 bytes per entry instead of 12.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -29,32 +35,39 @@ namespace {
 
 constexpr int CS_THREADS = 256;
 constexpr int CS_CAP = 2048;  // products staged per workgroup (16 KB LDS)
+constexpr int POS_MAX = 64;   // positional values: longest row
 
-template <typename PtrT, bool INDIRECT>
+// POS: va holds the POS_MAX-or-fewer positional values, not one per entry.
+template <typename PtrT, bool INDIRECT, bool POS = false>
 __global__ __launch_bounds__(CS_THREADS) void k_csr_stream(
     const PtrT *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ va,
     const double *x, double *y, const int *__restrict__ out_row, const int *__restrict__ blk,
     int blk0) {
   __shared__ double prod[CS_CAP];
+  __shared__ double pv[POS ? POS_MAX : 1];
   const int b = blk0 + blockIdx.x;
   const int r0 = blk[b], r1 = blk[b + 1];
   const PtrT j0 = rp[r0], j1 = rp[r1];
   const long len = (long)(j1 - j0);
   if (len <= CS_CAP) {
+    if constexpr (POS) {
+      if (threadIdx.x < POS_MAX) pv[threadIdx.x] = va[threadIdx.x];
+    }
     for (long t = threadIdx.x; t < len; t += CS_THREADS) {
       const PtrT j = j0 + (PtrT)t;
-      prod[t] = __dmul_rn(va[j], x[ci[j]]);
+      prod[t] = POS ? x[ci[j]] : __dmul_rn(va[j], x[ci[j]]);
     }
     __syncthreads();
     for (int r = r0 + (int)threadIdx.x; r < r1; r += CS_THREADS) {
       double s = 0.0;
-      const PtrT e = rp[r + 1];
-      for (PtrT j = rp[r]; j < e; j++) s = __dadd_rn(s, prod[j - j0]);
+      const PtrT jr = rp[r], e = rp[r + 1];
+      for (PtrT j = jr; j < e; j++)
+        s = __dadd_rn(s, POS ? __dmul_rn(pv[j - jr], prod[j - j0]) : prod[j - j0]);
       y[INDIRECT ? out_row[r] : r] = s;
     }
   } else if (threadIdx.x == 0) {  // a single row longer than the LDS stage
     double s = 0.0;
-    for (PtrT j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(va[j], x[ci[j]]));
+    for (PtrT j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(POS ? va[j - j0] : va[j], x[ci[j]]));
     y[INDIRECT ? out_row[r0] : r0] = s;
   }
 }
@@ -124,7 +137,8 @@ struct mpkx_plan {
   std::vector<char> gather_staged; // per phase: sources overlap destinations
   DevArray<long> rp;
   DevArray<int> ci, out_row, blk, gdst, gsrc, xdst, xsrc, csrc;
-  DevArray<double> va, vv, stage;
+  DevArray<double> va, vv, stage;  // va: one value per entry, or the positional values
+  bool positional = false;
   long rows = 0, nnz = 0, ghosts = 0;
   GraphCache graph;
 };
@@ -132,6 +146,10 @@ struct mpkx_plan {
 extern "C" {
 
 int mpkx_plan_create(mpkx_plan **out, mpkx_buffers *const *bufs, int npart) {
+  return mpkx_plan_create_opts(out, bufs, npart, 0);
+}
+
+int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart, int flags) {
   clear_error();
   if (!out || !bufs || npart <= 0) return fail(MPKX_EINVAL, "plan_create: bad arguments");
   const mpkx_buffers *b0 = bufs[0];
@@ -212,6 +230,31 @@ int mpkx_plan_create(mpkx_plan **out, mpkx_buffers *const *bufs, int npart) {
   }
   pl->rows = (long)rp.size() - 1;
   pl->nnz = (long)ci.size();
+  // Positional values: every row holds the same value (bitwise) at each
+  // position, rows of at most POS_MAX entries.
+  std::vector<double> posv;
+  if (!(flags & MPKX_PLAN_EXPLICIT) && pl->nnz > 0) {
+    std::vector<char> seen(POS_MAX, 0);
+    posv.assign(POS_MAX, 0.0);
+    bool ok = true;
+    for (long r = 0; r < pl->rows && ok; r++) {
+      if (rp[r + 1] - rp[r] > POS_MAX) ok = false;
+      for (long j = rp[r]; ok && j < rp[r + 1]; j++) {
+        const long k = j - rp[r];
+        if (!seen[k]) {
+          posv[k] = va[j];
+          seen[k] = 1;
+        } else if (memcmp(&posv[k], &va[j], sizeof(double)) != 0) {
+          ok = false;
+        }
+      }
+    }
+    pl->positional = ok;
+  }
+  if (pl->positional) {
+    va.swap(posv);
+    posv.clear();
+  }
 
   // Ghost gather of every phase >= 1: the Alltoallv of do_comm
   // (mpi2_mpkexecbufs_val.c:32-48) as (destination slot, source slot) pairs.
@@ -293,6 +336,15 @@ int mpkx_dcopy(double *dst, const double *src, long n, void *stream) {
   return MPKX_OK;
 }
 
+int mpkx_plan_info(const mpkx_plan *pl, int *positional, long *matrix_bytes) {
+  clear_error();
+  if (!pl) return fail(MPKX_EINVAL, "plan_info: null plan");
+  if (positional) *positional = pl->positional ? 1 : 0;
+  if (matrix_bytes)
+    *matrix_bytes = (long)(pl->rp.count * sizeof(long) + pl->ci.count * sizeof(int) + pl->va.count * sizeof(double));
+  return MPKX_OK;
+}
+
 int mpkx_plan_stats(const mpkx_plan *pl, long *rows, long *nnz, long *ghosts, int *launches) {
   clear_error();
   if (!pl) return fail(MPKX_EINVAL, "plan_stats: null plan");
@@ -330,8 +382,12 @@ static int run_phases(mpkx_plan *pl, hipStream_t st) {
     }
     for (; gi < pl->groups.size() && pl->groups[gi].ph == ph; gi++) {
       const auto &g = pl->groups[gi];
-      hipLaunchKernelGGL((k_csr_stream<long, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
-                         pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
+      if (pl->positional)
+        hipLaunchKernelGGL((k_csr_stream<long, true, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
+                           pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
+      else
+        hipLaunchKernelGGL((k_csr_stream<long, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
+                           pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
     }
   }
   MPKX_HIP(hipGetLastError());

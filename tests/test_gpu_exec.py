@@ -21,17 +21,32 @@ def per_power_rel_err(vv, ref, n, k):
     return np.array(out)
 
 
+def _positional(rp, va):
+    """True when every row holds the same value at each position."""
+    pos = np.arange(len(va)) - np.repeat(rp[:-1], np.diff(rp))
+    first = {}
+    for p, v in zip(pos, va):
+        if first.setdefault(p, v) != v:
+            return False
+    return True
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_exact_plan_bitwise_reference(case):
     d, rp, ci, va, parts, levs = O.case_inputs(case)
     say("exact plan", case)
     levels, sk, bufs, _ = M.prepare(rp, ci, va, d["nlevel"], d["nphase"], parts)
-    plan = M.Plan(bufs)
     x = O.uniform(d["seed"], d["n"])
-    vv = zeros_dev(d["n"] * (d["nlevel"] + 1))
-    plan.exec(to_dev(x), vv)
-    out = to_host(vv)
-    assert O.sha(out.tobytes()) == d["results_seeded.bin"]
+    # the positional plan (when the values are constant per row position,
+    # as for the cube's diagonal-first rows) and the plan with one value per
+    # entry both give the reference's bits
+    for flags, positional in ((0, _positional(rp, va)), (M.Plan.EXPLICIT, False)):
+        plan = M.Plan(bufs, flags)
+        assert plan.stats()["positional"] == positional
+        vv = zeros_dev(d["n"] * (d["nlevel"] + 1))
+        plan.exec(to_dev(x), vv)
+        out = to_host(vv)
+        assert O.sha(out.tobytes()) == d["results_seeded.bin"], flags
     # host drop-in: each rank's vv_buf[0:rcount[0]] loaded as the reference main does
     bufs.alloc_val_bufs()
     for p in range(d["npart"]):
@@ -325,9 +340,14 @@ def test_exact_plan_headline_size_bitwise_oracle():
     assert np.array_equal(to_host(vv), gold)
 
 
-def test_exact_plan_cube64_against_oracle():
-    say("exact plan cube64")
+@pytest.mark.parametrize("values", ["stencil", "random"])
+def test_exact_plan_cube64_against_oracle(values):
+    """Stencil values take the positional plan, random values (no value
+    constant per position) keep one value per entry; both bitwise."""
+    say("exact plan cube64", values)
     rp, ci, va = M.gen2("cube", 64, -1.0)
+    if values == "random":
+        va = va * (1 + 0.1 * np.random.default_rng(29).standard_normal(len(va)))
     n = len(rp) - 1
     k = 8
     x = M.uniform_pm1(0x5EED, n)
@@ -336,6 +356,7 @@ def test_exact_plan_cube64_against_oracle():
         parts = [M.partition_grid([64] * 3, [2, 2, 2], [16 * ph] * 3) for ph in range(max(nphase, 1))]
         _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
         plan = M.Plan(bufs)
+        assert plan.stats()["positional"] == (values == "stencil")
         for graph in (False, True):
             vv = zeros_dev(n * (k + 1))
             plan.exec(to_dev(x), vv, use_graph=graph)
