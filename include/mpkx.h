@@ -171,6 +171,23 @@ int mpkx_fill_buffers(int n, const int *ptr, const int *col,
                       mpkx_buffers ***out);
 void mpkx_del_bufs_array(mpkx_buffers **bufs, int npart);
 
+/* Prep and exec directories.  The reference keeps its prep output in a  */
+/* directory <head>_<npart>_<nlevel>_<nphase> holding loop.g0,            */
+/* g<ph>.part.<npart>, l<ph> and s<nphase>, with the values in            */
+/* <path up to its first '_'>.loop.val (mpk2/comm_data.c:27-169).        */
+/* mpkx_dir_params parses the name (read_dir, comm_data.c:27-43);        */
+/* mpkx_values_path forms the values path (alloc_read_val, :122-149);    */
+/* mpkx_wrtbufs reads the directory as new_comm_data does and writes     */
+/* <dir>/rank<r>.bufs of every rank, byte-equal to mpi2_mpkwrtbufs run on */
+/* npart ranks (mpk2/mpi2_mpkwrtbufs.c:43-63; with_values = 0 leaves out  */
+/* mval_buf); mpkx_write_results writes <dir>/results.txt (levels 0 ..    */
+/* nlevel-1, "%lf ") and results.bin (n*(nlevel+1) doubles) as            */
+/* write_results does (mpk2/mpi2_mpkexecbufs_val.c:58-75).                */
+int mpkx_dir_params(const char *dir, int *npart, int *nlevel, int *nphase);
+int mpkx_values_path(const char *dir, char *out, size_t size);
+int mpkx_wrtbufs(const char *dir, int with_values);
+int mpkx_write_results(const char *dir, int n, int nlevel, const double *vv);
+
 /* Shared-memory schedule of mpk2/mpkprep.c:7-252 (prep_mpk): task lists
  * of l*n+i per (phase, partition) then the skirt tasks, after the same
  * structure checks (MPKX_ESTRUCT when one fails).  tcount receives

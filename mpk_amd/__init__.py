@@ -92,6 +92,10 @@ def lib():
         "mpkx_prep_mpk": ([C.c_int, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P, P, P], C.c_int),
         "mpkx_plan_create": ([P, P, C.c_int], C.c_int),
         "mpkx_plan_create_opts": ([P, P, C.c_int, C.c_int], C.c_int),
+        "mpkx_dir_params": ([C.c_char_p, P, P, P], C.c_int),
+        "mpkx_values_path": ([C.c_char_p, C.c_char_p, C.c_size_t], C.c_int),
+        "mpkx_wrtbufs": ([C.c_char_p, C.c_int], C.c_int),
+        "mpkx_write_results": ([C.c_char_p, C.c_int, C.c_int, P], C.c_int),
         "mpkx_plan_info": ([P, P, P], C.c_int),
         "mpkx_plan_destroy": ([P], None),
         "mpkx_exec_bufs": ([P, P], C.c_int),
@@ -436,6 +440,32 @@ def fill_buffers(ptr, col, val, nlevel, nphase, parts, levels, skirt):
                                C.cast(pp, P), C.cast(lp, P), _p(skirt.sptr), _p(skirt.srow),
                                _p(skirt.sdist), skirt.npart, C.byref(out)))
     return BufferSet(out, skirt.npart), parts
+
+
+def dir_params(dirname):
+    """(npart, nlevel, nphase) from a prep directory name <head>_<npart>_<nlevel>_<nphase>."""
+    p, l, ph = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().mpkx_dir_params(dirname.encode(), C.byref(p), C.byref(l), C.byref(ph)))
+    return p.value, l.value, ph.value
+
+
+def values_path(dirname):
+    """Where the reference reads a prep directory's values (alloc_read_val's rule)."""
+    buf = C.create_string_buffer(4096)
+    _check(lib().mpkx_values_path(dirname.encode(), buf, len(buf)))
+    return buf.value.decode()
+
+
+def wrtbufs(dirname, with_values=True):
+    """mpi2_mpkwrtbufs on every rank: reads the prep directory, writes rank%d.bufs."""
+    _check(lib().mpkx_wrtbufs(dirname.encode(), int(with_values)))
+
+
+def write_results(dirname, vv, n, nlevel):
+    """results.txt and results.bin of a Krylov block (write_results' layout)."""
+    vv = np.ascontiguousarray(vv, np.float64)
+    assert vv.size == n * (nlevel + 1)
+    _check(lib().mpkx_write_results(dirname.encode(), n, nlevel, _p(vv)))
 
 
 def read_buffers(dirname, npart):

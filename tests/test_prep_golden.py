@@ -217,3 +217,61 @@ def test_c_abi_exports_every_declared_symbol():
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
     assert M.lib().mpkx_version().startswith(b"mpkx")
+
+
+def _prep_dir(case, root):
+    """A reference prep directory of a golden case: the graph and values as
+    gen2 writes them, the partition, level and skirt files of the case."""
+    import shutil
+    d = O.digests(case)
+    rp, ci, va = M.gen2(d["type"], d["size"])
+    path = os.path.join(root, case)
+    os.makedirs(path)
+    M.write_crs(f"{path}/loop.g0", rp, ci)
+    M.write_vals(M.values_path(path), va)
+    src = os.path.join(O.GOLDEN, case)
+    for f in os.listdir(src):
+        if re.fullmatch(r"g\d+\.part\.\d+|l\d+|s\d+", f):
+            shutil.copy(os.path.join(src, f), path)
+    return d, path, rp, ci, va
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_wrtbufs_directory_byte_equal_reference(case):
+    """mpi2_mpkwrtbufs on every rank in one call: the rank%d.bufs written from
+    a prep directory are the reference's bytes."""
+    import uuid
+    # the values path is cut at the first '_' of the whole path: no '_' above the directory
+    root = os.path.join(tempfile.gettempdir(), "mpkxdir" + uuid.uuid4().hex)
+    assert "_" not in root
+    os.makedirs(root)
+    try:
+        d, path, rp, ci, va = _prep_dir(case, root)
+        assert M.dir_params(path) == (d["npart"], d["nlevel"], d["nphase"])
+        assert M.values_path(path) == os.path.join(root, case.split("_")[0] + ".loop.val")
+        M.wrtbufs(path)
+        for r in range(d["npart"]):
+            b = open(f"{path}/rank{r}.bufs", "rb").read()
+            assert len(b) == d["bufs"][str(r)]["bytes"]
+            assert O.sha(b) == d["bufs"][str(r)]["sha256"], f"rank {r}"
+        # the same buffers read back and written as results of the oracle exec
+        x = O.uniform(d["seed"], d["n"])
+        vv = O.spmv_k(rp, ci, va, x, d["nlevel"])
+        M.write_results(path, vv, d["n"], d["nlevel"])
+        assert O.sha(open(f"{path}/results.bin", "rb").read()) == d["results_seeded.bin"]
+        lines = open(f"{path}/results.txt").read().split("\n")
+        assert len(lines) == d["nlevel"] + 1 and lines[-1] == ""
+        assert lines[1] == "".join("%f " % v for v in vv[d["n"]:2 * d["n"]])
+    finally:
+        import shutil
+        shutil.rmtree(root)
+
+
+def test_dir_params_errors():
+    with pytest.raises(M.MpkxError):
+        M.dir_params("/tmp/nodir")
+    with pytest.raises(M.MpkxError):
+        M.dir_params("/tmp/m5p8_4_x_1")
+    assert M.dir_params("/a/b_c/m5p8_4_6_2") == (4, 6, 2)
+    # alloc_read_val's rule cuts the whole path at its first '_'
+    assert M.values_path("/a/b_c/m5p8_4_6_2") == "/a/b.loop.val"
