@@ -187,6 +187,12 @@ int mpkx_dir_params(const char *dir, int *npart, int *nlevel, int *nphase);
 int mpkx_values_path(const char *dir, char *out, size_t size);
 int mpkx_wrtbufs(const char *dir, int with_values);
 int mpkx_write_results(const char *dir, int n, int nlevel, const double *vv);
+/* mpi2_mpkexecbufs_val run on npart ranks (mpk2/mpi2_mpkexecbufs_val.c:77-121),
+ * on the GPU: reads <dir>/rank<r>.bufs, loads rbuf[0] = x[idx_buf] (x on
+ * the host, n values; NULL: ones, as the reference loads), runs every
+ * phase, collects and writes results.txt / results.bin into dir; vv_out
+ * (host, n*(nlevel+1), may be NULL) receives the Krylov block too. */
+int mpkx_execbufs_dir(const char *dir, const double *x, double *vv_out);
 
 /* Shared-memory schedule of mpk2/mpkprep.c:7-252 (prep_mpk): task lists
  * of l*n+i per (phase, partition) then the skirt tasks, after the same

@@ -96,6 +96,7 @@ def lib():
         "mpkx_values_path": ([C.c_char_p, C.c_char_p, C.c_size_t], C.c_int),
         "mpkx_wrtbufs": ([C.c_char_p, C.c_int], C.c_int),
         "mpkx_write_results": ([C.c_char_p, C.c_int, C.c_int, P], C.c_int),
+        "mpkx_execbufs_dir": ([C.c_char_p, P, P], C.c_int),
         "mpkx_plan_info": ([P, P, P], C.c_int),
         "mpkx_plan_destroy": ([P], None),
         "mpkx_exec_bufs": ([P, P], C.c_int),
@@ -466,6 +467,20 @@ def write_results(dirname, vv, n, nlevel):
     vv = np.ascontiguousarray(vv, np.float64)
     assert vv.size == n * (nlevel + 1)
     _check(lib().mpkx_write_results(dirname.encode(), n, nlevel, _p(vv)))
+
+
+def execbufs_dir(dirname, x=None):
+    """mpi2_mpkexecbufs_val on every rank, on the GPU: reads rank%d.bufs of the
+    directory, runs with input x (None: ones), writes results.txt/.bin and
+    returns the Krylov block."""
+    npart, nlevel, nphase = dir_params(dirname)
+    b0 = read_buffers(dirname, 1)
+    n = b0[0].n
+    del b0
+    vv = np.zeros(n * (nlevel + 1))
+    xp = None if x is None else _p(np.ascontiguousarray(x, np.float64))
+    _check(lib().mpkx_execbufs_dir(dirname.encode(), xp, _p(vv)))
+    return vv
 
 
 def read_buffers(dirname, npart):

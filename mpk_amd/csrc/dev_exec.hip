@@ -445,6 +445,41 @@ int mpkx_exec(mpkx_plan *pl, const double *x, double *vv, void *stream, int use_
   return graph_run(pl->graph, x, vv, 0, st, [&](hipStream_t s) { return exec_launches(pl, x, vv, s) == MPKX_OK; });
 }
 
+// mpi2_mpkexecbufs_val on every rank (mpk2/mpi2_mpkexecbufs_val.c:77-121):
+// rank%d.bufs of the directory, input loaded as rbuf[0] = x[idx_buf] (the
+// reference loads ones), every phase on the GPU, collect_results, and
+// write_results into the directory.
+int mpkx_execbufs_dir(const char *dir, const double *x, double *vv_out) {
+  clear_error();
+  int npart, nlevel, nphase, rc;
+  if ((rc = mpkx_dir_params(dir, &npart, &nlevel, &nphase))) return rc;
+  std::vector<mpkx_buffers *> bufs(npart, nullptr);
+  struct Release {
+    std::vector<mpkx_buffers *> &b;
+    ~Release() {
+      for (auto *p : b)
+        if (p) mpkx_del_bufs(p);
+    }
+  } release{bufs};
+  for (int r = 0; r < npart; r++)
+    if (!(bufs[r] = mpkx_read_buffers(dir, r))) return MPKX_EIO;
+  const int n = bufs[0]->n;
+  if (bufs[0]->nlevel != nlevel || bufs[0]->nphase != nphase)
+    return fail(MPKX_EINVAL, "execbufs_dir: buffers of %s do not match its name", dir);
+  mpkx_plan *pl = nullptr;
+  if ((rc = mpkx_plan_create(&pl, bufs.data(), npart))) return rc;
+  std::unique_ptr<mpkx_plan> guard(pl);
+  std::vector<double> xh(n, 1.0), vh((size_t)n * (nlevel + 1));
+  if (x) std::copy(x, x + n, xh.begin());
+  DevArray<double> xd, vd;
+  if ((rc = xd.upload(xh)) || (rc = vd.alloc(vh.size()))) return rc;
+  if ((rc = mpkx_exec(pl, xd.p, vd.p, nullptr, 0))) return rc;
+  MPKX_HIP(hipMemcpy(vh.data(), vd.p, sizeof(double) * vh.size(), hipMemcpyDeviceToHost));
+  if ((rc = mpkx_write_results(dir, n, nlevel, vh.data()))) return rc;
+  if (vv_out) std::copy(vh.begin(), vh.end(), vv_out);
+  return MPKX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Plain k x SpMV (mpk2/verify_val.c:44-75)
 // ---------------------------------------------------------------------------

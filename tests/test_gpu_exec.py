@@ -510,3 +510,28 @@ def test_pdmpk_gpu_cube32():
         y2.zero_()
         plan.exec(xd, y2)
         assert np.array_equal(to_host(y2), out), rep
+
+
+@pytest.mark.parametrize("case", ["m5p8_4_4_1", "m9p16_8_6_2", "cube8_8_4_0", "cube16_8_8_1"])
+def test_execbufs_dir_reference_results(case):
+    """mpi2_mpkexecbufs_val on every rank from a prep directory: with the
+    seeded input the written results.bin is the reference exec's, with the
+    reference's own input (ones) it is verify_val's gold_result.bin."""
+    import os
+    import shutil
+    import tempfile
+    import uuid
+    from tests.test_prep_golden import _prep_dir
+    say("execbufs dir", case)
+    root = os.path.join(tempfile.gettempdir(), "mpkxdir" + uuid.uuid4().hex)
+    os.makedirs(root)
+    try:
+        d, path, rp, ci, va = _prep_dir(case, root)
+        M.wrtbufs(path)
+        vv = M.execbufs_dir(path, O.uniform(d["seed"], d["n"]))
+        assert O.sha(open(f"{path}/results.bin", "rb").read()) == d["results_seeded.bin"]
+        assert O.sha(vv.tobytes()) == d["results_seeded.bin"]
+        M.execbufs_dir(path)
+        assert O.sha(open(f"{path}/results.bin", "rb").read()) == d["gold_result.bin"]
+    finally:
+        shutil.rmtree(root)
