@@ -263,6 +263,12 @@ int mpkx_tasks_create(mpkx_tasks **t, int n, const int *ptr, const int *col,
 void mpkx_tasks_destroy(mpkx_tasks *t);
 /* x, vv device; vv[0..n) receives x, then every task l*n+i is computed. */
 int mpkx_tasks_exec(mpkx_tasks *t, const double *x, double *vv, void *stream);
+/* mpkrun (mpk2/mpkrun.c:49-74) on the GPU: reads the prep directory as
+ * read_mpk does (graph in <dir>/g0, values in <path up to the first
+ * '_'>.val), runs prep_mpk and the valued task exec with input x (host, n;
+ * NULL: ones), and writes results.txt / results.bin as mpkx_write_results;
+ * vv_out (host, may be NULL) receives the Krylov block. */
+int mpkx_mpkrun_dir(const char *dir, const double *x, double *vv_out);
 
 /* ------------------------------------------------------------------ */
 /* Plain k x SpMV baseline (mpk2/verify_val.c:44-75): vv[(l+1)n ..] =   */

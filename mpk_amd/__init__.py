@@ -97,6 +97,7 @@ def lib():
         "mpkx_wrtbufs": ([C.c_char_p, C.c_int], C.c_int),
         "mpkx_write_results": ([C.c_char_p, C.c_int, C.c_int, P], C.c_int),
         "mpkx_execbufs_dir": ([C.c_char_p, P, P], C.c_int),
+        "mpkx_mpkrun_dir": ([C.c_char_p, P, P], C.c_int),
         "mpkx_plan_info": ([P, P, P], C.c_int),
         "mpkx_plan_destroy": ([P], None),
         "mpkx_exec_bufs": ([P, P], C.c_int),
@@ -480,6 +481,17 @@ def execbufs_dir(dirname, x=None):
     vv = np.zeros(n * (nlevel + 1))
     xp = None if x is None else _p(np.ascontiguousarray(x, np.float64))
     _check(lib().mpkx_execbufs_dir(dirname.encode(), xp, _p(vv)))
+    return vv
+
+
+def mpkrun_dir(dirname, n, x=None):
+    """mpkrun on the GPU: prep_mpk task lists from the prep directory (graph
+    in g0), valued exec with input x (None: ones), results.txt/.bin written;
+    returns the Krylov block."""
+    npart, nlevel, nphase = dir_params(dirname)
+    vv = np.zeros(n * (nlevel + 1))
+    xp = None if x is None else _p(np.ascontiguousarray(x, np.float64))
+    _check(lib().mpkx_mpkrun_dir(dirname.encode(), xp, _p(vv)))
     return vv
 
 

@@ -643,6 +643,41 @@ int mpkx_tasks_exec(mpkx_tasks *t, const double *x, double *vv, void *stream) {
   return MPKX_OK;
 }
 
+// mpkrun (mpk2/mpkrun.c:49-74) on the GPU: read_mpk of the directory (graph
+// in g0), the values at <path up to the first '_'>.val, prep_mpk, the
+// valued task exec (vexec_mpk) with input x (NULL: ones, as mpkrun loads)
+// and write_results.
+int mpkx_mpkrun_dir(const char *dir, const double *x, double *vv_out) {
+  clear_error();
+  PrepDir pd;
+  int rc;
+  if ((rc = load_prep_dir(dir, "g0", ".val", true, pd))) return rc;
+  const int n = pd.n, nph = pd.nphase > 0 ? pd.nphase : 1;
+  std::vector<const int *> pp(nph), lp(pd.nphase);
+  for (int ph = 0; ph < nph; ph++) pp[ph] = pd.parts[ph].data();
+  for (int ph = 0; ph < pd.nphase; ph++) lp[ph] = pd.levels[ph].data();
+  std::vector<int> tcount((size_t)(pd.nphase + 1) * pd.npart);
+  long *tidx = nullptr, ntask = 0;
+  if ((rc = mpkx_prep_mpk(n, pd.ptr, pd.col, pd.nlevel, pd.nphase, pp.data(), pd.nphase ? lp.data() : nullptr,
+                          pd.sptr, pd.srow, pd.sdist, pd.npart, tcount.data(), &tidx, &ntask)))
+    return rc;
+  std::unique_ptr<long, void (*)(void *)> tidx_guard(tidx, mpkx_free);
+  mpkx_tasks *T = nullptr;
+  if ((rc = mpkx_tasks_create(&T, n, pd.ptr, pd.col, pd.val.data(), pd.nlevel, pd.nphase, pd.npart, tcount.data(),
+                              tidx)))
+    return rc;
+  std::unique_ptr<mpkx_tasks, void (*)(mpkx_tasks *)> tguard(T, mpkx_tasks_destroy);
+  std::vector<double> xh(n, 1.0), vh((size_t)n * (pd.nlevel + 1));
+  if (x) std::copy(x, x + n, xh.begin());
+  DevArray<double> xd, vd;
+  if ((rc = xd.upload(xh)) || (rc = vd.alloc(vh.size()))) return rc;
+  if ((rc = mpkx_tasks_exec(T, xd.p, vd.p, nullptr))) return rc;
+  MPKX_HIP(hipMemcpy(vh.data(), vd.p, sizeof(double) * vh.size(), hipMemcpyDeviceToHost));
+  if ((rc = mpkx_write_results(dir, n, pd.nlevel, vh.data()))) return rc;
+  if (vv_out) std::copy(vh.begin(), vh.end(), vv_out);
+  return MPKX_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -12,7 +12,9 @@
 // per MPI rank and writes <dir>/rank<r>.bufs (mpk2/mpi2_mpkwrtbufs.c:43-63);
 // mpi2_mpkexecbufs_val writes <dir>/results.txt and results.bin
 // (mpk2/mpi2_mpkexecbufs_val.c:58-75).  Here one call does the prep of
-// every rank.
+// every rank.  The shared-memory programs use the same directory with the
+// graph in g0 and the values in <path up to the first '_'>.val
+// (read_mpk, mpk2/mpkread.c:49-185; alloc_read_val, mpk2/mpkrun.c:12-32).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,18 +24,6 @@
 #include "internal.h"
 
 using namespace mpkx;
-
-namespace {
-
-// Frees what the readers allocate.
-struct Owned {
-  std::vector<void *> p;
-  ~Owned() {
-    for (void *q : p) mpkx_free(q);
-  }
-};
-
-}  // namespace
 
 extern "C" {
 
@@ -67,50 +57,70 @@ int mpkx_values_path(const char *dir, char *out, size_t size) {
   return MPKX_OK;
 }
 
+}  // extern "C"
+
+namespace mpkx {
+
+PrepDir::~PrepDir() {
+  mpkx_free(ptr);
+  mpkx_free(col);
+  mpkx_free(sptr);
+  mpkx_free(srow);
+  mpkx_free(sdist);
+}
+
+int load_prep_dir(const char *dir, const char *graph_file, const char *values_suffix, bool with_values,
+                  PrepDir &pd) {
+  int rc;
+  if ((rc = mpkx_dir_params(dir, &pd.npart, &pd.nlevel, &pd.nphase))) return rc;
+  const std::string d(dir);
+  if ((rc = mpkx_read_crs((d + "/" + graph_file).c_str(), &pd.n, &pd.ptr, &pd.col))) return rc;
+  const int n = pd.n, nph = pd.nphase > 0 ? pd.nphase : 1;
+  pd.parts.assign(nph, std::vector<int>(n));
+  pd.levels.assign(pd.nphase, std::vector<int>(n));
+  for (int ph = 0; ph < nph; ph++) {
+    const std::string f = d + "/g" + std::to_string(ph) + ".part." + std::to_string(pd.npart);
+    if ((rc = mpkx_read_ints(f.c_str(), n, pd.parts[ph].data(), nullptr))) return rc;
+  }
+  for (int ph = 0; ph < pd.nphase; ph++) {
+    const std::string f = d + "/l" + std::to_string(ph);
+    if ((rc = mpkx_read_ints(f.c_str(), n, pd.levels[ph].data(), nullptr))) return rc;
+  }
+  if ((rc = mpkx_read_skirt((d + "/s" + std::to_string(pd.nphase)).c_str(), n, pd.npart, pd.nlevel, &pd.sptr,
+                            &pd.srow, &pd.sdist)))
+    return rc;
+  if (with_values) {
+    // the path up to its first '_', then the suffix
+    const char *u = strchr(dir, '_');
+    if (!u) return fail(MPKX_EINVAL, "no '_' in %s", dir);
+    const std::string vpath = std::string(dir, (size_t)(u - dir)) + values_suffix;
+    pd.val.resize((size_t)pd.ptr[n]);
+    if ((rc = mpkx_read_vals(vpath.c_str(), pd.ptr[n], pd.val.data()))) return rc;
+  }
+  return MPKX_OK;
+}
+
+}  // namespace mpkx
+
+extern "C" {
+
 int mpkx_wrtbufs(const char *dir, int with_values) {
   clear_error();
-  int npart, nlevel, nphase, rc;
-  if ((rc = mpkx_dir_params(dir, &npart, &nlevel, &nphase))) return rc;
-  const std::string d(dir);
-  Owned own;
-  int n = 0, *ptr = nullptr, *col = nullptr;
-  if ((rc = mpkx_read_crs((d + "/loop.g0").c_str(), &n, &ptr, &col))) return rc;
-  own.p.push_back(ptr);
-  own.p.push_back(col);
-  const int nph = nphase > 0 ? nphase : 1;
-  std::vector<std::vector<int>> parts(nph, std::vector<int>(n)), levels(nphase, std::vector<int>(n));
-  for (int ph = 0; ph < nph; ph++) {
-    const std::string f = d + "/g" + std::to_string(ph) + ".part." + std::to_string(npart);
-    if ((rc = mpkx_read_ints(f.c_str(), n, parts[ph].data(), nullptr))) return rc;
-  }
-  for (int ph = 0; ph < nphase; ph++) {
-    const std::string f = d + "/l" + std::to_string(ph);
-    if ((rc = mpkx_read_ints(f.c_str(), n, levels[ph].data(), nullptr))) return rc;
-  }
-  long *sptr = nullptr;
-  int *srow = nullptr, *sdist = nullptr;
-  if ((rc = mpkx_read_skirt((d + "/s" + std::to_string(nphase)).c_str(), n, npart, nlevel, &sptr, &srow, &sdist)))
-    return rc;
-  own.p.push_back(sptr);
-  own.p.push_back(srow);
-  own.p.push_back(sdist);
-  std::vector<double> val;
-  if (with_values) {
-    char vpath[4096];
-    if ((rc = mpkx_values_path(dir, vpath, sizeof vpath))) return rc;
-    val.resize((size_t)ptr[n]);
-    if ((rc = mpkx_read_vals(vpath, ptr[n], val.data()))) return rc;
-  }
+  PrepDir pd;
+  int rc;
+  if ((rc = load_prep_dir(dir, "loop.g0", ".loop.val", with_values != 0, pd))) return rc;
+  const int nph = pd.nphase > 0 ? pd.nphase : 1;
   std::vector<int *> pp(nph);
-  std::vector<const int *> lp(nphase);
-  for (int ph = 0; ph < nph; ph++) pp[ph] = parts[ph].data();
-  for (int ph = 0; ph < nphase; ph++) lp[ph] = levels[ph].data();
+  std::vector<const int *> lp(pd.nphase);
+  for (int ph = 0; ph < nph; ph++) pp[ph] = pd.parts[ph].data();
+  for (int ph = 0; ph < pd.nphase; ph++) lp[ph] = pd.levels[ph].data();
   mpkx_buffers **bufs = nullptr;
-  if ((rc = mpkx_fill_buffers(n, ptr, col, with_values ? val.data() : nullptr, nlevel, nphase, pp.data(),
-                              nphase ? lp.data() : nullptr, sptr, srow, sdist, npart, &bufs)))
+  if ((rc = mpkx_fill_buffers(pd.n, pd.ptr, pd.col, with_values ? pd.val.data() : nullptr, pd.nlevel, pd.nphase,
+                              pp.data(), pd.nphase ? lp.data() : nullptr, pd.sptr, pd.srow, pd.sdist, pd.npart,
+                              &bufs)))
     return rc;
-  for (int r = 0; r < npart && rc == MPKX_OK; r++) rc = mpkx_write_buffers(bufs[r], dir);
-  mpkx_del_bufs_array(bufs, npart);
+  for (int r = 0; r < pd.npart && rc == MPKX_OK; r++) rc = mpkx_write_buffers(bufs[r], dir);
+  mpkx_del_bufs_array(bufs, pd.npart);
   return rc;
 }
 

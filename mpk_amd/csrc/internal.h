@@ -51,6 +51,25 @@ struct SkirtView {
   const int *sdist;
 };
 
+// A prep directory <head>_<npart>_<nlevel>_<nphase> read as the
+// reference reads it (host_dirs.cpp): the graph file, the partition of
+// every phase (one with nphase = 0), the levels of phases < nphase, the
+// skirt, and optionally the values at <path up to the first '_'> + suffix.
+struct PrepDir {
+  int n = 0, npart = 0, nlevel = 0, nphase = 0;
+  int *ptr = nullptr, *col = nullptr;
+  std::vector<std::vector<int>> parts, levels;
+  long *sptr = nullptr;
+  int *srow = nullptr, *sdist = nullptr;
+  std::vector<double> val;
+  PrepDir() = default;
+  PrepDir(const PrepDir &) = delete;
+  PrepDir &operator=(const PrepDir &) = delete;
+  ~PrepDir();
+};
+int load_prep_dir(const char *dir, const char *graph_file, const char *values_suffix, bool with_values,
+                  PrepDir &pd);
+
 }  // namespace mpkx
 
 #define MPKX_TRY_ALLOC(expr)                                            \

@@ -535,3 +535,28 @@ def test_execbufs_dir_reference_results(case):
         assert O.sha(open(f"{path}/results.bin", "rb").read()) == d["gold_result.bin"]
     finally:
         shutil.rmtree(root)
+
+
+@pytest.mark.parametrize("case", ["m5p8_4_4_0", "m7p12_4_4_1", "cube8_8_4_1"])
+def test_mpkrun_dir_reference_results(case):
+    """mpkrun from a prep directory (graph in g0, values in <head>.val): the
+    prep_mpk task lists run on the GPU give the reference exec's results."""
+    import os
+    import shutil
+    import tempfile
+    import uuid
+    from tests.test_prep_golden import _prep_dir
+    say("mpkrun dir", case)
+    root = os.path.join(tempfile.gettempdir(), "mpkxdir" + uuid.uuid4().hex)
+    os.makedirs(root)
+    try:
+        d, path, rp, ci, va = _prep_dir(case, root)
+        shutil.copy(f"{path}/loop.g0", f"{path}/g0")
+        shutil.copy(M.values_path(path), os.path.join(root, case.split("_")[0] + ".val"))
+        vv = M.mpkrun_dir(path, d["n"], O.uniform(d["seed"], d["n"]))
+        assert O.sha(vv.tobytes()) == d["results_seeded.bin"]
+        assert O.sha(open(f"{path}/results.bin", "rb").read()) == d["results_seeded.bin"]
+        M.mpkrun_dir(path, d["n"])
+        assert O.sha(open(f"{path}/results.bin", "rb").read()) == d["gold_result.bin"]
+    finally:
+        shutil.rmtree(root)
