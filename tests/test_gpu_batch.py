@@ -210,7 +210,7 @@ def test_mlp_adam_trains():
 
 def test_cost_model_gain_on_held_out():
     """The committed model's picks on held-out matrices (a suite seed not used
-    for training) against the fixed default configuration, both judged by
+    for training) against the fixed default configuration, both timed by
     the exhaustive autotuner."""
     path = os.path.join(REPO, "mpk_amd", "cost_model.json")
     if not os.path.exists(path):
