@@ -67,6 +67,43 @@ __device__ __forceinline__ void store_rhs(float *p, const float (&v)[B]) {
   }
 }
 
+// Entries j0, j0 + S, ..., j0 + (U-1)*S (those below e) of a row into acc,
+// in that order.  The column and value loads of all U entries are issued
+// first, then their x loads, then the multiply-adds: a lane keeps up to U
+// gathers in flight instead of waiting for each column index before its x
+// load.  Loads of entries past the end are masked off (no memory traffic).
+// entries per lane per batch: fewer for wide right-hand-side blocks, whose
+// x loads already hold B registers each
+template <int B>
+constexpr int unr() { return B >= 16 ? 2 : 4; }
+template <int B, int U, int S>
+__device__ __forceinline__ void accumulate(int j0, int e, const int *__restrict__ ci, const float *__restrict__ va,
+                                           const float *xc, int ldv, float (&acc)[B]) {
+  int cj[U];
+  float aj[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    cj[u] = 0;
+    aj[u] = 0.0f;
+    if (j0 + u * S < e) {
+      cj[u] = ci[j0 + u * S];
+      aj[u] = va[j0 + u * S];
+    }
+  }
+  float xv[U][B];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+#pragma unroll
+    for (int b = 0; b < B; b++) xv[u][b] = 0.0f;
+    if (j0 + u * S < e) load_rhs<B>(xc + (size_t)cj[u] * ldv, xv[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (j0 + u * S < e)
+#pragma unroll
+      for (int b = 0; b < B; b++) acc[b] = fmaf(aj[u], xv[u][b], acc[b]);
+}
+
 // L lanes per row; lane l sums entries rp[row] + l, + L, ... for B
 // right-hand sides held in registers; the L partial sums meet in a
 // butterfly of lane shuffles.
@@ -88,13 +125,7 @@ __global__ __launch_bounds__(BT) void k_spmm_group(const int *__restrict__ rows,
   if (active) {
     const int e = rp[row + 1];
     const float *xc = x + c0;
-    for (int j = rp[row] + l; j < e; j += L) {
-      const float a = va[j];
-      float xv[B];
-      load_rhs<B>(xc + (size_t)ci[j] * ldv, xv);
-#pragma unroll
-      for (int b = 0; b < B; b++) acc[b] = fmaf(a, xv[b], acc[b]);
-    }
+    for (int j0 = rp[row] + l; j0 < e; j0 += unr<B>() * L) accumulate<B, unr<B>(), L>(j0, e, ci, va, xc, ldv, acc);
   }
 #pragma unroll
   for (int off = L / 2; off >= 1; off >>= 1)
@@ -117,13 +148,8 @@ __global__ __launch_bounds__(BT) void k_spmm_long(const int *__restrict__ rows, 
 #pragma unroll
   for (int b = 0; b < B; b++) acc[b] = 0.0f;
   const float *xc = x + c0;
-  for (int j = rp[row] + t; j < rp[row + 1]; j += BT) {
-    const float a = va[j];
-    float xv[B];
-    load_rhs<B>(xc + (size_t)ci[j] * ldv, xv);
-#pragma unroll
-    for (int b = 0; b < B; b++) acc[b] = fmaf(a, xv[b], acc[b]);
-  }
+  const int e = rp[row + 1];
+  for (int j0 = rp[row] + t; j0 < e; j0 += unr<B>() * BT) accumulate<B, unr<B>(), BT>(j0, e, ci, va, xc, ldv, acc);
 #pragma unroll
   for (int b = 0; b < B; b++) red[b][t] = acc[b];
   __syncthreads();
