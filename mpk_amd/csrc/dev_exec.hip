@@ -36,6 +36,7 @@ namespace {
 constexpr int CS_THREADS = 256;
 constexpr int CS_CAP = 2048;  // products staged per workgroup (16 KB LDS)
 constexpr int POS_MAX = 64;   // positional values: longest row
+constexpr int CS_BATCH = 8;   // entries per thread in flight while staging
 
 // POS: va holds the POS_MAX-or-fewer positional values, not one per entry.
 template <typename PtrT, bool INDIRECT, bool POS = false>
@@ -53,9 +54,31 @@ __global__ __launch_bounds__(CS_THREADS) void k_csr_stream(
     if constexpr (POS) {
       if (threadIdx.x < POS_MAX) pv[threadIdx.x] = va[threadIdx.x];
     }
-    for (long t = threadIdx.x; t < len; t += CS_THREADS) {
-      const PtrT j = j0 + (PtrT)t;
-      prod[t] = POS ? x[ci[j]] : __dmul_rn(va[j], x[ci[j]]);
+    // CS_BATCH entries per thread in flight: column (and value) loads,
+    // then the x loads, then the products; entries past len are masked off
+    for (long t0 = threadIdx.x; t0 < len; t0 += CS_BATCH * CS_THREADS) {
+      int cj[CS_BATCH];
+      double av[CS_BATCH], xv[CS_BATCH];
+#pragma unroll
+      for (int u = 0; u < CS_BATCH; u++) {
+        const long t = t0 + (long)u * CS_THREADS;
+        cj[u] = 0;
+        av[u] = 0.0;
+        if (t < len) {
+          cj[u] = ci[j0 + (PtrT)t];
+          if constexpr (!POS) av[u] = va[j0 + (PtrT)t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CS_BATCH; u++) {
+        xv[u] = 0.0;
+        if (t0 + (long)u * CS_THREADS < len) xv[u] = x[cj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < CS_BATCH; u++) {
+        const long t = t0 + (long)u * CS_THREADS;
+        if (t < len) prod[t] = POS ? xv[u] : __dmul_rn(av[u], xv[u]);
+      }
     }
     __syncthreads();
     for (int r = r0 + (int)threadIdx.x; r < r1; r += CS_THREADS) {

@@ -24,6 +24,7 @@ namespace {
 
 constexpr int PT = 256;
 constexpr int PCAP = 2048;
+constexpr int PBATCH = 8;  // entries per thread in flight while staging
 
 // CSR-stream row kernel with an optional partial-sum seed per row.
 __global__ __launch_bounds__(PT) void k_pd_rows(const long *__restrict__ rp, const int *__restrict__ ci,
@@ -36,7 +37,30 @@ __global__ __launch_bounds__(PT) void k_pd_rows(const long *__restrict__ rp, con
   const long j0 = rp[r0], j1 = rp[r1];
   const long len = j1 - j0;
   if (len <= PCAP) {
-    for (long t = threadIdx.x; t < len; t += PT) prod[t] = __dmul_rn(va[j0 + t], mb[ci[j0 + t]]);
+    // PBATCH entries per thread in flight: column and value loads, then the
+    // mbuf loads, then the products; entries past len are masked off
+    for (long t0 = threadIdx.x; t0 < len; t0 += PBATCH * PT) {
+      int cj[PBATCH];
+      double av[PBATCH], xv[PBATCH];
+#pragma unroll
+      for (int u = 0; u < PBATCH; u++) {
+        const long t = t0 + (long)u * PT;
+        cj[u] = 0;
+        av[u] = 0.0;
+        if (t < len) {
+          cj[u] = ci[j0 + t];
+          av[u] = va[j0 + t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PBATCH; u++) {
+        xv[u] = 0.0;
+        if (t0 + (long)u * PT < len) xv[u] = mb[cj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < PBATCH; u++)
+        if (t0 + (long)u * PT < len) prod[t0 + (long)u * PT] = __dmul_rn(av[u], xv[u]);
+    }
     __syncthreads();
     for (int r = r0 + (int)threadIdx.x; r < r1; r += PT) {
       double s = seed[r] >= 0 ? mb[seed[r]] : 0.0;
