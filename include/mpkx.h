@@ -312,7 +312,8 @@ int mpkx_fused_create(mpkx_fused **f, int n, const int *ptr, const int *col,
                       const double *val, const int *part, int npart,
                       int nlevel, int s, int flags);
 void mpkx_fused_destroy(mpkx_fused *f);
-/* x, vv device; vv[0..n) receives x.  use_graph as in mpkx_spmv_k. */
+/* x, vv device; vv[0..n) receives x (x == vv: level 0 already holds x
+ * and is read in place).  use_graph as in mpkx_spmv_k. */
 int mpkx_fused_exec(mpkx_fused *f, const double *x, double *vv, void *stream,
                     int use_graph);
 /* Launches one round only: levels L0+1 .. L0+w with L0 = round * s

@@ -44,8 +44,10 @@ static void launch_round(mpkx_fused *f, double *vv, int L0, const double *xin, h
                      P.maxslots);
 }
 
+// x == vv: level 0 already holds x, and the first round reads it in place.
 static void fused_launches(mpkx_fused *f, const double *x, double *vv, hipStream_t st) {
-  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s) launch_round(f, vv, L0, L0 == 0 ? x : nullptr, st);
+  for (int L0 = 0; L0 < f->plan.nlevel; L0 += f->plan.s)
+    launch_round(f, vv, L0, L0 == 0 && x != vv ? x : nullptr, st);
 }
 
 extern "C" {

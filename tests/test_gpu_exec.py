@@ -99,6 +99,11 @@ def test_fused_dict_bitwise_oracle(s):
         F.exec(to_dev(x), vv, use_graph=graph)
         out = to_host(vv)
         assert np.array_equal(out, gold), per_power_rel_err(out, gold, n, k)
+        # x already in level 0 of the block (x aliases vv): read in place
+        vv2 = zeros_dev(n * (k + 1))
+        vv2[:n] = to_dev(x)
+        F.exec(vv2[:n], vv2, use_graph=graph)
+        assert np.array_equal(to_host(vv2), gold)
 
 
 @pytest.mark.parametrize("typ,size,blocks,k,s", [
