@@ -1,6 +1,8 @@
-"""The C ABI from a plain C caller (tests/c_abi/exec_bufs_example.c): the
+"""The C ABI from plain C callers: tests/c_abi/exec_bufs_example.c (the
 mpk2 prep, the rank%d.bufs round trip and, on a GPU, mpkx_exec_bufs for all
-ranks, checked in C against k x SpMV."""
+ranks, checked in C against k x SpMV) and tests/c_abi/prep_dir_example.c
+(the reference's file-based flow: a prep directory, mpkx_wrtbufs and, on a
+GPU, mpkx_execbufs_dir, checked in C against verify_val's k x SpMV)."""
 import os
 import subprocess
 
@@ -8,12 +10,24 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(REPO, "tests", "c_abi", "exec_bufs_example")
+DIR_EXE = os.path.join(REPO, "tests", "c_abi", "prep_dir_example")
 
 
-def _exe():
-    if not os.path.exists(EXE):
+def _exe(path=EXE):
+    if not os.path.exists(path):
         pytest.skip("C example not built (run __graft_entry__.build())")
-    return EXE
+    return path
+
+
+def _root():
+    """A scratch directory whose path has no '_' (the reference's values
+    path is cut at the first '_' of the whole path)."""
+    import tempfile
+    import uuid
+    root = os.path.join(tempfile.gettempdir(), "mpkxc" + uuid.uuid4().hex)
+    assert "_" not in root
+    os.makedirs(root)
+    return root
 
 
 def test_c_caller_prep(tmp_path):
@@ -29,3 +43,29 @@ def test_c_caller_exec_on_gpu(tmp_path):
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     assert "exec ok" in r.stdout
+
+
+def test_c_caller_prep_directory():
+    import shutil
+    root = _root()
+    try:
+        r = subprocess.run([_exe(DIR_EXE), root], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "prep ok" in r.stdout
+        d = os.path.join(root, "cube16_8_6_1")
+        assert sorted(f for f in os.listdir(d) if f.endswith(".bufs")) == [f"rank{i}.bufs" for i in range(8)]
+    finally:
+        shutil.rmtree(root)
+
+
+@pytest.mark.gpu
+def test_c_caller_exec_directory_on_gpu():
+    import shutil
+    root = _root()
+    try:
+        r = subprocess.run([_exe(DIR_EXE), root, "--gpu"], capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0, r.stderr
+        assert "exec ok" in r.stdout
+    finally:
+        shutil.rmtree(root)
