@@ -306,6 +306,11 @@ enum {
   MPKX_FUSED_BLOCK512 = 64,
   MPKX_FUSED_BLOCK1024 = 96,
   MPKX_FUSED_BLOCK_MASK = 96,
+  /* one LDS buffer: a level's new values wait in registers until the     */
+  /* level's reads are done (one more barrier per level); half the LDS per */
+  /* tile, so twice as many workgroups share a CU; also accepted by        */
+  /* mpkx_fused32_create OR-ed into values                                 */
+  MPKX_FUSED_ONEBUF = 128,
 };
 /* part[n] labels in [0, npart); s = levels per launch (1..8). */
 int mpkx_fused_create(mpkx_fused **f, int n, const int *ptr, const int *col,

@@ -11,7 +11,15 @@ namespace {
 
 using K64 = FusedKernel<double, double>;
 
-K64 pick64(int T, int MR, int W, int nd) {
+K64 pick64(int T, int MR, int W, int nd, bool onebuf) {
+  if (onebuf) {
+    if (T == 256 && MR == 4) return pick_fused_w<double, double, 256, 4, true>(W, nd);
+    if (T == 256 && MR == 7) return pick_fused_w<double, double, 256, 7, true>(W, nd);
+    if (T == 512 && MR == 4) return pick_fused_w<double, double, 512, 4, true>(W, nd);
+    if (T == 512 && MR == 7) return pick_fused_w<double, double, 512, 7, true>(W, nd);
+    if (T == 1024 && MR == 4) return pick_fused_w<double, double, 1024, 4, true>(W, nd);
+    return nullptr;
+  }
   if (T == 256 && MR == 4) return pick_fused_w<double, double, 256, 4>(W, nd);
   if (T == 256 && MR == 8) return pick_fused_w<double, double, 256, 8>(W, nd);
   if (T == 512 && MR == 4) return pick_fused_w<double, double, 512, 4>(W, nd);
@@ -27,6 +35,7 @@ struct mpkx_fused {
   double dv[4] = {0, 0, 0, 0};
   size_t lds_bytes = 0;
   long slots = 0, matrix_bytes = 0;
+  bool onebuf = false;  // one LDS buffer (MPKX_FUSED_ONEBUF)
   K64 fn = nullptr;
   DevArray<int> hdr, grow;
   DevArray<uint32_t> ent;
@@ -61,9 +70,12 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   std::unique_ptr<mpkx_fused> guard(f);
   int rc;
   try {
-    std::vector<std::pair<int, int>> mr = block_choices(flags & MPKX_FUSED_BLOCK_MASK, {{512, 4}, {1024, 4}, {1024, 8}});
+    f->onebuf = (flags & MPKX_FUSED_ONEBUF) != 0;
+    std::vector<std::pair<int, int>> mr =
+        block_choices(flags & MPKX_FUSED_BLOCK_MASK,
+                      f->onebuf ? onebuf_choices() : std::vector<std::pair<int, int>>{{512, 4}, {1024, 4}, {1024, 8}});
     rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, !(flags & MPKX_FUSED_EXPLICIT),
-                         2 * (int)sizeof(double), mr, f->plan,
+                         (f->onebuf ? 1 : 2) * (int)sizeof(double), mr, f->plan,
                          (flags & MPKX_FUSED_PRODUCTS) ? 1 : (flags & MPKX_FUSED_SELECT) ? 2
                          : (flags & MPKX_FUSED_POSITIONAL) ? 3 : 0);
   } catch (const std::bad_alloc &) {
@@ -72,9 +84,9 @@ int mpkx_fused_create(mpkx_fused **out, int n, const int *ptr, const int *col, c
   if (rc) return rc;
   TilePlan &P = f->plan;
   for (size_t i = 0; i < P.dictv.size(); i++) f->dv[i] = P.dictv[i];
-  f->fn = pick64(P.T, P.MR, P.W, P.ndict);
+  f->fn = pick64(P.T, P.MR, P.W, P.ndict, f->onebuf);
   if (!f->fn) return fail(MPKX_EUNSUPPORTED, "fused_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(double) * nd_planes(P.ndict);  // + the zero record
+  f->lds_bytes = (size_t)(f->onebuf ? 1 : 2) * (P.maxslots + 1) * sizeof(double) * nd_planes(P.ndict);  // + the zero record
   f->slots = P.region_slots;
   f->matrix_bytes = (long)P.ent.size() * 4 + (long)P.vals.size() * 8;
   if (P.vals.empty()) P.vals.push_back(0.0);

@@ -18,7 +18,15 @@ using K32f = FusedKernel<float, float>;
 using K32b = FusedKernel<float, uint16_t>;
 
 template <typename MT>
-FusedKernel<float, MT> pick32(int T, int MR, int W, int nd) {
+FusedKernel<float, MT> pick32(int T, int MR, int W, int nd, bool onebuf) {
+  if (onebuf) {
+    if (T == 256 && MR == 4) return pick_fused_w<float, MT, 256, 4, true>(W, nd);
+    if (T == 256 && MR == 7) return pick_fused_w<float, MT, 256, 7, true>(W, nd);
+    if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4, true>(W, nd);
+    if (T == 512 && MR == 7) return pick_fused_w<float, MT, 512, 7, true>(W, nd);
+    if (T == 1024 && MR == 4) return pick_fused_w<float, MT, 1024, 4, true>(W, nd);
+    return nullptr;
+  }
   if (T == 256 && MR == 4) return pick_fused_w<float, MT, 256, 4>(W, nd);
   if (T == 256 && MR == 8) return pick_fused_w<float, MT, 256, 8>(W, nd);
   if (T == 512 && MR == 4) return pick_fused_w<float, MT, 512, 4>(W, nd);
@@ -47,6 +55,7 @@ struct mpkx_fused32 {
   double sigma = 1.0;
   size_t lds_bytes = 0;
   long slots = 0, matrix_bytes = 0;
+  bool onebuf = false;  // one LDS buffer (MPKX_FUSED_ONEBUF)
   K32f fnf = nullptr;
   K32b fnb = nullptr;
   DevArray<int> hdr, grow;
@@ -81,7 +90,8 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
                         const int *part, int npart, int nlevel, int s, int values, double *sigma_out) {
   clear_error();
   const int block = values & MPKX_FUSED_BLOCK_MASK;
-  values &= ~MPKX_FUSED_BLOCK_MASK;
+  const bool onebuf = (values & MPKX_FUSED_ONEBUF) != 0;
+  values &= ~(MPKX_FUSED_BLOCK_MASK | MPKX_FUSED_ONEBUF);
   if (!out || values < 0 || values > 5) return fail(MPKX_EINVAL, "fused32_create: bad arguments");
   const bool dict_ok = values == MPKX_VALUES_AUTO || values == MPKX_VALUES_AUTO_PRODUCTS ||
                        values == MPKX_VALUES_AUTO_SELECT || values == MPKX_VALUES_AUTO_POSITIONAL;
@@ -90,11 +100,14 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
   auto *f = new (std::nothrow) mpkx_fused32;
   if (!f) return fail(MPKX_ENOMEM, "fused32_create");
   std::unique_ptr<mpkx_fused32> guard(f);
+  f->onebuf = onebuf;
   int rc;
   try {
-    rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, dict_ok, 2 * (int)sizeof(float),
-                         block_choices(block, {{512, 4}, {1024, 4}, {1024, 8}, {1024, 12}}), f->plan,
-                         dict_codec);
+    rc = build_tile_plan(n, ptr, col, val, part, npart, nlevel, s, dict_ok, (onebuf ? 1 : 2) * (int)sizeof(float),
+                         block_choices(block, onebuf ? onebuf_choices()
+                                                     : std::vector<std::pair<int, int>>{{512, 4}, {1024, 4}, {1024, 8},
+                                                                                        {1024, 12}}),
+                         f->plan, dict_codec);
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fused32_create: host allocation");
   }
@@ -118,13 +131,13 @@ int mpkx_fused32_create(mpkx_fused32 **out, int n, const int *ptr, const int *co
   f->inv_sigma = (float)(1.0 / f->sigma);
   if (sigma_out) *sigma_out = f->sigma;
   if (f->values == 2) {
-    f->fnb = pick32<uint16_t>(P.T, P.MR, P.W, 0);
+    f->fnb = pick32<uint16_t>(P.T, P.MR, P.W, 0, f->onebuf);
     if (!f->fnb) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   } else {
-    f->fnf = pick32<float>(P.T, P.MR, P.W, P.ndict);
+    f->fnf = pick32<float>(P.T, P.MR, P.W, P.ndict, f->onebuf);
     if (!f->fnf) return fail(MPKX_EUNSUPPORTED, "fused32_create: no kernel for T=%d MR=%d W=%d", P.T, P.MR, P.W);
   }
-  f->lds_bytes = (size_t)2 * (P.maxslots + 1) * sizeof(float) * nd_planes(P.ndict);  // + the zero record
+  f->lds_bytes = (size_t)(f->onebuf ? 1 : 2) * (P.maxslots + 1) * sizeof(float) * nd_planes(P.ndict);  // + the zero record
   f->slots = P.region_slots;
   std::vector<float> vf;
   std::vector<uint16_t> vb;

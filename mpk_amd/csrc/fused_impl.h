@@ -233,7 +233,12 @@ __device__ __forceinline__ void load_rows(const TileRef &t, int tid, uint32_t (&
 // -2, -4 = select codec with 2 or 4 dictionary values; ND_POS* =
 // positional values.  inv_sigma scales every new power (power of two:
 // exact); FP64 passes 1.  One workgroup per tile.
-template <typename VT, typename MT, int T, int MR, int W, int ND>
+//
+// ONEBUF: one LDS buffer instead of two.  A level's new values wait in
+// registers until every read of the level is done (one more barrier per
+// level), so a tile needs half the LDS and twice as many workgroups fit on
+// a CU.
+template <typename VT, typename MT, int T, int MR, int W, int ND, bool ONEBUF = false>
 __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const int *__restrict__ grow,
                                              const uint32_t *__restrict__ ent, const MT *__restrict__ vals,
                                              VT d0, VT d1, VT d2, VT d3, VT inv_sigma, VT *vv,
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   // more register rows: the extra registers would cost occupancy.
   constexpr bool XREG = MR <= 4 && MR <= GB;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = buf0 + (size_t)(maxslots + 1) * NP;
+  VT *buf0 = reinterpret_cast<VT *>(lds_raw), *buf1 = ONEBUF ? buf0 : buf0 + (size_t)(maxslots + 1) * NP;
   const int tid = threadIdx.x;
   const VT *src = vv + (size_t)L0 * n;
   const TileRef t = tile_ref(hdr, grow, ent, xcd_remap(blockIdx.x, ntiles), w);
@@ -305,6 +310,7 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
   __syncthreads();
 
   VT *in = buf0, *out = buf1;
+  VT sv[MR];  // ONEBUF without XREG: the level's new values until its reads are done
   for (int j = 1; j <= w; j++) {
     const int nact = t.cnt[w - j];
     VT *dst = vv + (size_t)(L0 + j) * n;
@@ -360,12 +366,24 @@ __global__ __launch_bounds__(T) void k_fused(const int *__restrict__ hdr, const 
         }
         if (SCALE) s = s * inv_sigma;
         if constexpr (XREG) xr[m] = s;
-        put_slot<VT, ND>(out, r, maxslots + 1, s, d0, d1, d2, d3);
+        if constexpr (!ONEBUF) {
+          put_slot<VT, ND>(out, r, maxslots + 1, s, d0, d1, d2, d3);
+        } else if constexpr (!XREG) {
+          sv[m] = s;
+        }
         if (own[m] >= 0) st_at(dst, (uint32_t)sizeof(VT) * (uint32_t)own[m], s);
       }
       // keep the next row's LDS loads below this point: one row's operands
       // live at a time instead of MR*W of them
       __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (ONEBUF) {  // every read of this level is done: overwrite level L0+j-1
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < MR; m++) {
+        const int r = m * T + tid;
+        if (r < nact) put_slot<VT, ND>(buf0, r, maxslots + 1, XREG ? xr[m] : sv[m], d0, d1, d2, d3);
+      }
     }
     __syncthreads();
     VT *tmp = in;
@@ -378,26 +396,26 @@ template <typename VT, typename MT>
 using FusedKernel = void (*)(const int *, const int *, const uint32_t *, const MT *, VT, VT, VT, VT, VT,
                              VT *, const double *, int, int, int, int, int);
 
-template <typename VT, typename MT, int T, int MR, int W>
+template <typename VT, typename MT, int T, int MR, int W, bool ONEBUF>
 FusedKernel<VT, MT> pick_fused_nd(int nd) {
   if constexpr (MR * ((W + 1) / 2) > 64) {  // beyond the plan builder's register budget
     return nullptr;
   } else if constexpr (std::is_same<MT, uint16_t>::value) {  // BF16 values always stream
-    return nd == 0 ? k_fused<VT, MT, T, MR, W, 0> : nullptr;
+    return nd == 0 ? k_fused<VT, MT, T, MR, W, 0, ONEBUF> : nullptr;
   } else {
     switch (nd) {
-      case 0: return k_fused<VT, MT, T, MR, W, 0>;
-      case 1: return k_fused<VT, MT, T, MR, W, 1>;
-      case 2: return k_fused<VT, MT, T, MR, W, 2>;
-      case 4: return k_fused<VT, MT, T, MR, W, 4>;
-      case -2: return k_fused<VT, MT, T, MR, W, -2>;
-      case -4: return k_fused<VT, MT, T, MR, W, -4>;
-      case ND_POS: return k_fused<VT, MT, T, MR, W, ND_POS>;
-      case ND_POS_NEG1: return k_fused<VT, MT, T, MR, W, ND_POS_NEG1>;
-      case ND_POS_POS1: return k_fused<VT, MT, T, MR, W, ND_POS_POS1>;
-      case ND_POSD: return k_fused<VT, MT, T, MR, W, ND_POSD>;
-      case ND_POSD + 1: return k_fused<VT, MT, T, MR, W, ND_POSD + 1>;
-      case ND_POSD + 2: return k_fused<VT, MT, T, MR, W, ND_POSD + 2>;
+      case 0: return k_fused<VT, MT, T, MR, W, 0, ONEBUF>;
+      case 1: return k_fused<VT, MT, T, MR, W, 1, ONEBUF>;
+      case 2: return k_fused<VT, MT, T, MR, W, 2, ONEBUF>;
+      case 4: return k_fused<VT, MT, T, MR, W, 4, ONEBUF>;
+      case -2: return k_fused<VT, MT, T, MR, W, -2, ONEBUF>;
+      case -4: return k_fused<VT, MT, T, MR, W, -4, ONEBUF>;
+      case ND_POS: return k_fused<VT, MT, T, MR, W, ND_POS, ONEBUF>;
+      case ND_POS_NEG1: return k_fused<VT, MT, T, MR, W, ND_POS_NEG1, ONEBUF>;
+      case ND_POS_POS1: return k_fused<VT, MT, T, MR, W, ND_POS_POS1, ONEBUF>;
+      case ND_POSD: return k_fused<VT, MT, T, MR, W, ND_POSD, ONEBUF>;
+      case ND_POSD + 1: return k_fused<VT, MT, T, MR, W, ND_POSD + 1, ONEBUF>;
+      case ND_POSD + 2: return k_fused<VT, MT, T, MR, W, ND_POSD + 2, ONEBUF>;
     }
     return nullptr;
   }
@@ -405,28 +423,38 @@ FusedKernel<VT, MT> pick_fused_nd(int nd) {
 
 // nd: 0 value stream, 1/2/4 product records, -2/-4 select codec, ND_POS*
 // positional values.
-template <typename VT, typename MT, int T, int MR>
+template <typename VT, typename MT, int T, int MR, bool ONEBUF = false>
 FusedKernel<VT, MT> pick_fused_w(int W, int nd) {
   switch (W) {
-    case 5: return pick_fused_nd<VT, MT, T, MR, 5>(nd);
-    case 7: return pick_fused_nd<VT, MT, T, MR, 7>(nd);
-    case 9: return pick_fused_nd<VT, MT, T, MR, 9>(nd);
-    case 16: return pick_fused_nd<VT, MT, T, MR, 16>(nd);
+    case 5: return pick_fused_nd<VT, MT, T, MR, 5, ONEBUF>(nd);
+    case 7: return pick_fused_nd<VT, MT, T, MR, 7, ONEBUF>(nd);
+    case 9: return pick_fused_nd<VT, MT, T, MR, 9, ONEBUF>(nd);
+    case 16: return pick_fused_nd<VT, MT, T, MR, 16, ONEBUF>(nd);
   }
   return nullptr;
 }
 
 // (threads per workgroup, register rows per thread) pairs the plan builder
 // may pick from, smallest first: MPKX_FUSED_BLOCK256 / 512 / 1024 restrict
-// them to one workgroup size (256 adds 256x4 and 256x8), else dflt.
+// dflt to one workgroup size (256x4 and 256x8 when dflt has no 256).
 inline std::vector<std::pair<int, int>> block_choices(int block_flags,
                                                       const std::vector<std::pair<int, int>> &dflt) {
-  if (block_flags == MPKX_FUSED_BLOCK256) return {{256, 4}, {256, 8}};
+  const int want = block_flags == MPKX_FUSED_BLOCK256    ? 256
+                   : block_flags == MPKX_FUSED_BLOCK512  ? 512
+                   : block_flags == MPKX_FUSED_BLOCK1024 ? 1024
+                                                         : 0;
   std::vector<std::pair<int, int>> out;
-  const int want = block_flags == MPKX_FUSED_BLOCK512 ? 512 : block_flags == MPKX_FUSED_BLOCK1024 ? 1024 : 0;
   for (auto &c : dflt)
     if (!want || c.first == want) out.push_back(c);
+  if (out.empty() && want == 256) out = {{256, 4}, {256, 8}};
   return out;
+}
+
+// Choices with one LDS buffer: smaller workgroups, so that the halved LDS
+// lets more of them share a CU.
+inline const std::vector<std::pair<int, int>> &onebuf_choices() {
+  static const std::vector<std::pair<int, int>> c{{256, 4}, {256, 7}, {512, 4}, {512, 7}, {1024, 4}};
+  return c;
 }
 
 // Host-side tile plan.

@@ -51,6 +51,8 @@ def time_fused(F, xd, vv, reps, warmup=2):
 CODECS = {"auto": (0, 0), "products": (2, 3), "select": (4, 4), "positional": (8, 5)}
 for _b, _f in (("256", 32), ("512", 64), ("1024", 96)):
     CODECS.update({c + "+b" + _b: (f | _f, v | _f) for c, (f, v) in list(CODECS.items()) if "+" not in c})
+# "+1buf": one LDS buffer (any of the above)
+CODECS.update({c + "+1buf": (f | 128, v | 128) for c, (f, v) in list(CODECS.items())})
 
 
 def tune_grid(ptr, col, val, dims, nlevel, precision="f64", candidates=None, reps=10, x=None, log=None,

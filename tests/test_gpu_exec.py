@@ -246,6 +246,53 @@ def test_fused_block_sizes_bitwise_oracle(codec, block):
         assert np.array_equal(to_host(vv), gold), (codec, block, s)
 
 
+@pytest.mark.parametrize("codec", ["auto", "select", "products", "explicit"])
+@pytest.mark.parametrize("block", [0, 256, 512])
+def test_fused_one_lds_buffer_bitwise_oracle(codec, block):
+    """One LDS buffer (new values held in registers until a level's reads
+    are done) gives the reference's bits for every codec and s."""
+    say("fused one buffer", codec, block)
+    N, k = 48, 7
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    if codec == "explicit":
+        va = va * (1 + 0.1 * np.random.default_rng(31).standard_normal(len(va)))
+    n = len(rp) - 1
+    part = M.partition_grid([N] * 3, [6, 6, 6])
+    x = M.uniform_pm1(37, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
+    flags = M.FusedMPK.ONEBUF | {0: 0, 256: M.FusedMPK.BLOCK256, 512: M.FusedMPK.BLOCK512}[block]
+    flags |= {"auto": 0, "select": M.FusedMPK.SELECT, "products": M.FusedMPK.PRODUCTS, "explicit": 0}[codec]
+    for s in (1, 2, 3):
+        F = M.FusedMPK(rp, ci, va, part, k, s, flags)
+        assert F.stats()["dict_codec"] == {"auto": "positional", "select": "select", "products": "products",
+                                           "explicit": None}[codec]
+        if block:
+            assert F.stats()["block"] == block
+        vv = zeros_dev(n * (k + 1))
+        F.exec(to_dev(x), vv)
+        assert np.array_equal(to_host(vv), gold), (codec, block, s)
+
+
+def test_fused32_one_lds_buffer_agrees():
+    """FP32 with one LDS buffer gives the same bits as with two."""
+    import torch
+    say("fused32 one buffer")
+    N, k = 64, 8
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    n = len(rp) - 1
+    part = M.partition_grid([N] * 3, [8, 8, 8])
+    x = to_dev(M.uniform_pm1(0x5EED, n))
+    outs = []
+    for values in (M.FusedMPK32.AUTO, M.FusedMPK32.AUTO | M.FusedMPK.ONEBUF,
+                   M.FusedMPK32.FP32 | M.FusedMPK.ONEBUF | M.FusedMPK.BLOCK256):
+        F = M.FusedMPK32(rp, ci, va, part, k, 3, values)
+        vv = torch.zeros(n * (k + 1), dtype=torch.float32, device="cuda")
+        F.exec(x, vv)
+        outs.append(to_host(vv))
+    assert np.array_equal(outs[0], outs[1])
+    assert np.allclose(outs[0], outs[2], rtol=1e-6, atol=1e-7)  # FP32 values: fma instead of dictionary products
+
+
 def test_fused32_block_sizes_agree():
     """FP32 rounds give the same bits for every workgroup size (the order of
     operations per row does not depend on it)."""
