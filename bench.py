@@ -142,11 +142,12 @@ def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C, c
         mats = list(pool.map(lambda i: M.gen_chunglu(int(ns[i]), 10.0, 2.5, 1.0, int(seeds[i])), range(count)))
     t_gen = time.perf_counter() - t0
     bm = M.BatchMPK(mats, nrhs, k)
+    dflt = bm.config  # mpkx_batch_default_config over all rows
     rows, nnz = bm.rows, bm.stats()["nnz"]
     X = torch.rand(rows * nrhs, device=dev, dtype=torch.float32) * 2 - 1
     VV = torch.empty(rows * nrhs * (k + 1), device=dev, dtype=torch.float32)
     t0 = time.perf_counter()
-    times = CM.tune(bm, X, VV, reps=3, configs=[tuple(config), CM.default_config(nrhs)] if config else None)
+    times = CM.tune(bm, X, VV, reps=3, configs=[tuple(config), dflt] if config else None)
     t_tune = time.perf_counter() - t0
     best = tuple(config) if config else min(times, key=times.get)
     bm.configure(*best)
@@ -170,7 +171,7 @@ def batched_form(M, torch, dev, count=256, nrhs=8, k=6, steps=10, seed=0xBA7C, c
             "ms_per_step": round(ms, 5), "value": round(gbs, 2), "unit": "GB/s",
             "pct_hbm_roofline": round(100 * gbs / HBM_PEAK_GBS, 2),
             "bytes_per_power": bpp, "config": list(best), "autotuned": config is None,
-            "default_config_ms": round(times[CM.default_config(nrhs)], 5),
+            "default_config": list(dflt), "default_config_ms": round(times[dflt], 5),
             "configs_tried": len(times), "tune_seconds": round(t_tune, 2), "generate_seconds": round(t_gen, 2),
             "launches_in_graph": bm.stats()["launches"], "long_rows": bm.stats()["long_rows"],
             "per_power_rel_err_vs_fp64_smallest_matrix": errs}
@@ -190,13 +191,13 @@ def learned_tuning(M, torch, dev, path, n=1500000, nrhs=16, k=10, seed=0xC5):
     pick = CM.nearest(pred, nrhs)
     t_model = time.perf_counter() - t0
     bm = M.BatchMPK([(rp, ci, va)], nrhs, k)
+    dflt = bm.config  # mpkx_batch_default_config
     X = torch.rand(n * nrhs, device=dev, dtype=torch.float32) * 2 - 1
     VV = torch.empty(n * nrhs * (k + 1), device=dev, dtype=torch.float32)
     t0 = time.perf_counter()
     times = CM.tune(bm, X, VV, reps=3)
     t_sweep = time.perf_counter() - t0
     best = min(times, key=times.get)
-    dflt = CM.default_config(nrhs)
     rank = sorted(times.values()).index(times[pick]) + 1
     return {"workload": f"Chung-Lu n={n}, nnz={int(rp[-1])}, {nrhs} RHS, k={k}, FP32",
             "model_pick": list(pick), "model_ms": round(times[pick], 5),

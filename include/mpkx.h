@@ -454,10 +454,26 @@ void mpkx_batch_destroy(mpkx_batch *b);
 /* each lane's registers (power of two <= 16 dividing nrhs),            */
 /* lanes_per_row lanes splitting each row's entries (power of two       */
 /* <= 64), rows with more than long_row entries get a workgroup each.   */
-/* Default (min(nrhs,16), 4, 32): the best single configuration over  */
-/* the synthetic tuning suite.                                          */
+/* mpkx_batch_create applies mpkx_batch_default_config over the rows  */
+/* of all its matrices.                                                 */
 int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row,
                          int long_row);
+/* The configuration in effect.                                         */
+int mpkx_batch_config(const mpkx_batch *b, int *block_width,
+                      int *lanes_per_row, int *long_row);
+/* Default configuration from row statistics (host only, no GPU), for  */
+/* nmat matrices whose nrows rows ptr describes (concatenated):         */
+/* block width min(nrhs, 16); lanes per row 8 when the row lengths'     */
+/* coefficient of variation exceeds 0.5, else 2 when the mean row       */
+/* length is at most 12, else 4; long-row threshold 32 unless more than */
+/* 30% of the rows have more than 32 entries, then 128 unless more than */
+/* 30% have more than 256, then 512; at least 128 when nmat > 1 and the */
+/* coefficient of variation exceeds 0.5.  Fitted to exhaustive timings  */
+/* of 2200 synthetic matrices and 300 batches (stencils, banded,        */
+/* Chung-Lu).                                                           */
+int mpkx_batch_default_config(int nmat, long nrows, const int *ptr,
+                              int nrhs, int *block_width,
+                              int *lanes_per_row, int *long_row);
 /* X: device FP32 rows*nrhs; VV: device FP32 rows*nrhs*(nlevel+1) (may  */
 /* begin at X, then the copy of level 0 is skipped).                    */
 int mpkx_batch_exec(mpkx_batch *b, const float *X, float *VV, void *stream,

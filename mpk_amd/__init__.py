@@ -138,6 +138,8 @@ def lib():
         "mpkx_batch_create": ([P, C.c_int, P, P, P, P, C.c_int, C.c_int, P], C.c_int),
         "mpkx_batch_destroy": ([P], None),
         "mpkx_batch_configure": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
+        "mpkx_batch_config": ([P, P, P, P], C.c_int),
+        "mpkx_batch_default_config": ([C.c_int, C.c_long, P, C.c_int, P, P, P], C.c_int),
         "mpkx_batch_exec": ([P, P, P, P, C.c_int], C.c_int),
         "mpkx_batch_stats": ([P, P, P, P, P], C.c_int),
         "mpkx_features": ([C.c_int, P, P, C.c_int, P], C.c_int),
@@ -802,6 +804,17 @@ def gen_banded(n, bandwidth, per_row, diag_shift=1.0, seed=1):
     return _take_csr(n, rp, ci, va)
 
 
+def batch_default_config(ptr, nrhs, nmat=1):
+    """mpkx_batch_default_config: (block width, lanes per row, long-row
+    threshold) from the row lengths of a CSR row pointer (the rows of nmat
+    matrices, concatenated; host only)."""
+    ptr = _i32(ptr)
+    B, L, T = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().mpkx_batch_default_config(nmat, len(ptr) - 1, _p(ptr), nrhs, C.byref(B), C.byref(L),
+                                           C.byref(T)))
+    return B.value, L.value, T.value
+
+
 class BatchMPK:
     """Batched block-RHS MPK over many matrices in one hipGraph launch.
 
@@ -824,7 +837,9 @@ class BatchMPK:
                                        _p(self.sigmas)))
         self.row_offsets = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
         self.rows = int(self.row_offsets[-1])
-        self.config = (min(nrhs, 16), 4, 32)
+        B, L, T = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().mpkx_batch_config(self._h, C.byref(B), C.byref(L), C.byref(T)))
+        self.config = (B.value, L.value, T.value)
 
     def configure(self, block_width, lanes_per_row, long_row):
         _check(lib().mpkx_batch_configure(self._h, block_width, lanes_per_row, long_row))

@@ -29,12 +29,23 @@ import mpk_amd as M
 WIDTHS = (1, 2, 4, 8, 16)
 LANES = (1, 2, 4, 8, 16, 32)
 LONG_ROWS = (32, 128, 512, 2048)
-DEFAULT = (16, 4, 32)  # mpkx_batch_create's configuration (B = min(nrhs, 16))
 TAU = 0.03
 
 
-def default_config(nrhs):
-    return (min(nrhs, DEFAULT[0]), DEFAULT[1], DEFAULT[2])
+def default_config(nrhs, feats, nmat=1):
+    """mpkx_batch_create's configuration (mpkx_batch_default_config) from the
+    features of nmat matrices' rows: mean row length (2), its coefficient of
+    variation (3) and the fractions of rows longer than 32 (9) and 256 (10)."""
+    mean, cv, big32, big256 = feats[2], feats[3], feats[9], feats[10]
+    lanes = 8 if cv > 0.5 else 2 if mean <= 12.0 else 4
+    long_row = 32 if big32 <= 0.3 else 128 if big256 <= 0.3 else 512
+    if nmat > 1 and cv > 0.5:
+        long_row = max(long_row, 128)
+    return (min(nrhs, 16), lanes, long_row)
+
+
+def spec_nmat(spec):
+    return len(spec["ns"]) if spec["kind"] == "batch" else 1
 
 
 def space(nrhs):
@@ -109,7 +120,59 @@ def suite(count, seed=1):
     return out
 
 
+def batch_suite(count, seed=21):
+    """Specs of `count` synthetic batches for mpkx_batch (block-diagonal sets
+    of small matrices): 8..512 matrices of one kind (stencil, banded,
+    Chung-Lu), rows per matrix log-uniform in [1k, 200k], at most 8M rows."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        sub = ("stencil", "banded", "chunglu")[i % 3]
+        nrhs = int(rng.choice([4, 8, 16], p=[0.2, 0.3, 0.5]))
+        nmat = int(round(math.exp(rng.uniform(math.log(8), math.log(512)))))
+        lo = math.exp(rng.uniform(math.log(1000), math.log(50000)))
+        hi = min(lo * math.exp(rng.uniform(0, math.log(40))), 200000)
+        ns = np.exp(rng.uniform(math.log(lo), math.log(hi), nmat)).astype(int)
+        ns = ns[:max(1, int(np.searchsorted(np.cumsum(ns), 8_000_000)))]
+        s = {"kind": "batch", "sub": sub, "nrhs": nrhs, "ns": [int(v) for v in ns],
+             "seed": int(rng.integers(1, 2**62))}
+        if sub == "stencil":
+            s["type"] = str(rng.choice(["m5p", "m9p", "m7p", "cube"]))
+        elif sub == "banded":
+            s.update(bandwidth=int(math.exp(rng.uniform(math.log(10), math.log(1e3)))),
+                     per_row=int(rng.integers(3, 34)))
+        else:
+            s.update(exponent=float(rng.uniform(2.1, 3.0)), mean=float(rng.uniform(4.0, 30.0)))
+        out.append(s)
+    return out
+
+
+def _batch_member(spec, i):
+    n, seed = spec["ns"][i], (spec["seed"] + i) % 2**62 + 1
+    if spec["sub"] == "stencil":
+        side = max(8, int(round(n ** (1 / 3 if spec["type"] == "cube" else 1 / 2))))
+        return M.gen2(spec["type"], side, -1.0)
+    if spec["sub"] == "banded":
+        return M.gen_banded(n, min(spec["bandwidth"], n - 1), spec["per_row"], 1.0, seed)
+    return M.gen_chunglu(n, spec["mean"], spec["exponent"], 1.0, seed)
+
+
+def concat(mats):
+    """Block-diagonal CSR (ptr, col) of a list of (ptr, col, val): the rows
+    mpkx_batch_create concatenates (features of a batch)."""
+    if len(mats) == 1:
+        return mats[0][0], mats[0][1]
+    rows = np.cumsum([0] + [len(m[0]) - 1 for m in mats])
+    ents = np.cumsum([0] + [int(m[0][-1]) for m in mats])
+    ptr = np.concatenate([[0]] + [m[0][1:].astype(np.int64) + e for m, e in zip(mats, ents[:-1])])
+    col = np.concatenate([m[1].astype(np.int64) + r for m, r in zip(mats, rows[:-1])])
+    return ptr.astype(np.int32), col.astype(np.int32)
+
+
 def make_matrix(spec):
+    """(ptr, col, val) of a suite spec; a list of them for a batch spec."""
+    if spec["kind"] == "batch":
+        return [_batch_member(spec, i) for i in range(len(spec["ns"]))]
     if spec["kind"] == "stencil":
         return M.gen2(spec["type"], spec["size"], -1.0)
     if spec["kind"] == "banded":
@@ -136,11 +199,13 @@ def collect(specs, nlevel=2, reps=3, threads=8, log=None, checkpoint=None):
     feats, timings = [], []
     t0 = time.time()
     with ThreadPoolExecutor(threads) as pool:
-        for i, (spec, (rp, ci, va)) in enumerate(zip(specs, generate(specs, pool, 2 * threads))):
+        for i, (spec, mat) in enumerate(zip(specs, generate(specs, pool, 2 * threads))):
             nrhs = spec["nrhs"]
-            feats.append(M.features(rp, ci, nrhs))
-            bm = M.BatchMPK([(rp, ci, va)], nrhs, nlevel)
-            n = len(rp) - 1
+            mats = mat if isinstance(mat, list) else [mat]
+            feats.append(M.features(*concat(mats), nrhs))
+            bm = M.BatchMPK(mats, nrhs, nlevel)
+            del mat, mats
+            n = bm.rows
             X = torch.rand(n * nrhs, device="cuda:0", dtype=torch.float32) * 2 - 1
             VV = torch.empty(n * nrhs * (nlevel + 1), device="cuda:0", dtype=torch.float32)
             timings.append(tune(bm, X, VV, reps=reps))
@@ -258,6 +323,19 @@ def regret(timings, picks):
     return np.array([t[p] / min(t.values()) for t, p in zip(timings, picks)])
 
 
+def evaluate(model, specs, feats, timings, fixed):
+    """Mean and max regret of the model's picks, the library default and the
+    best fixed configuration per nrhs (`fixed`) over the given samples."""
+    picks = [nearest(p, s["nrhs"]) for p, s in zip(model.predict(feats), specs)]
+    defaults = [default_config(s["nrhs"], f, spec_nmat(s)) for s, f in zip(specs, feats)]
+    fixed_picks = [fixed[s["nrhs"]] for s in specs]
+    rm, rd, rf = regret(timings, picks), regret(timings, defaults), regret(timings, fixed_picks)
+    return {"n": len(specs), "model_mean_regret": float(rm.mean()), "model_max_regret": float(rm.max()),
+            "default_mean_regret": float(rd.mean()), "default_max_regret": float(rd.max()),
+            "best_fixed_mean_regret": float(rf.mean()), "best_fixed_max_regret": float(rf.max()),
+            "model_within_5pct": float(np.mean(rm <= 1.05))}
+
+
 # ---- data files and command line -----------------------------------------------
 
 def save_data(path, specs, feats, timings):
@@ -288,6 +366,14 @@ def main(argv=None):
     c.add_argument("--seed", type=int, default=1)
     c.add_argument("--start", type=int, default=0)
     c.add_argument("--out", required=True)
+    c.add_argument("--batches", action="store_true", help="the batch suite (batch_suite) instead")
+    e = sub.add_parser("evaluate", help="regret of a trained model on timings (collected, or fresh with --seed)")
+    e.add_argument("--model", required=True)
+    e.add_argument("--data", nargs="*", default=[])
+    e.add_argument("--count", type=int, default=200)
+    e.add_argument("--seed", type=int, default=None, help="collect a fresh suite of --count matrices first")
+    e.add_argument("--batches", action="store_true", help="the fresh suite is a batch suite")
+    e.add_argument("--out", default=None, help="where to save the fresh timings")
     t = sub.add_parser("train", help="train the MLP on collected timings (GPU)")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
@@ -299,10 +385,29 @@ def main(argv=None):
         print(msg, flush=True)
 
     if a.cmd == "collect":
-        specs = suite(a.count, a.seed)[a.start:]
+        specs = (batch_suite if a.batches else suite)(a.count, a.seed)[a.start:]
         feats, timings = collect(specs, log=log, checkpoint=lambda sp, f, t: save_data(a.out, sp, f, t))
         save_data(a.out, specs, feats, timings)
         log(f"wrote {a.out}: {len(specs)} matrices")
+        return
+    if a.cmd == "evaluate":
+        model = CostModel.load(a.model)
+        if a.seed is not None:
+            specs = (batch_suite if a.batches else suite)(a.count, a.seed)
+            feats, timings = collect(specs, log=log)
+            if a.out:
+                save_data(a.out, specs, feats, timings)
+        else:
+            specs, feats, timings = [], [], []
+            for p in a.data:
+                s, f, t = load_data(p)
+                specs += s
+                feats.append(f)
+                timings += t
+            feats = np.concatenate(feats)
+        fixed = {int(r): tuple(c) for r, c in model.meta["evaluation"]["best_fixed_configs"].items()}
+        res = evaluate(model, specs, feats, timings, fixed)
+        log(json.dumps(res))
         return
     specs, feats, timings = [], [], []
     for p in a.data:
@@ -320,16 +425,7 @@ def main(argv=None):
     for name, sel in (("train", tr), ("holdout", hold)):
         if len(sel) == 0:
             continue
-        tm = [timings[i] for i in sel]
-        picks = [nearest(p, specs[i]["nrhs"]) for p, i in zip(model.predict(feats[sel]), sel)]
-        defaults = [default_config(specs[i]["nrhs"]) for i in sel]
-        fixed_picks = [fixed[specs[i]["nrhs"]] for i in sel]
-        rm, rd, rf = regret(tm, picks), regret(tm, defaults), regret(tm, fixed_picks)
-        out[name] = {"n": int(len(sel)), "model_mean_regret": float(rm.mean()),
-                     "model_max_regret": float(rm.max()), "default_mean_regret": float(rd.mean()),
-                     "default_max_regret": float(rd.max()), "best_fixed_mean_regret": float(rf.mean()),
-                     "best_fixed_max_regret": float(rf.max()),
-                     "model_within_5pct": float(np.mean(rm <= 1.05))}
+        out[name] = evaluate(model, [specs[i] for i in sel], feats[sel], [timings[i] for i in sel], fixed)
         log(f"{name}: {out[name]}")
     model.meta.update({"evaluation": out, "loss_history": hist, "data": [str(p) for p in a.data]})
     model.save(a.out)

@@ -234,6 +234,49 @@ static bool batch_launches(mpkx_batch *b, const float *X, float *VV, hipStream_t
 
 extern "C" {
 
+int mpkx_batch_default_config(int nmat, long nrows, const int *ptr, int nrhs, int *block_width,
+                              int *lanes_per_row, int *long_row) {
+  clear_error();
+  if (nmat <= 0 || nrows <= 0 || !ptr || !pow2(nrhs) || nrhs > 64)
+    return fail(MPKX_EINVAL, "batch_default_config: bad arguments");
+  // Row-length mean, coefficient of variation and the fractions of rows
+  // with more than 32 and more than 256 entries (features 2, 3, 9 and 10
+  // of mpkx_features).  The rule was fitted to the exhaustive timings of
+  // 2200 single synthetic matrices and 300 batches (stencils, banded,
+  // Chung-Lu): varied rows want 8 lanes per row, short uniform rows 2, the
+  // rest 4; the long-row threshold stays at 32 unless that makes more than
+  // 30% of the rows long.  In a batch of several power-law matrices most
+  // rows above 32 entries are small hubs whose gathers stay inside their
+  // own matrix, and a workgroup per such row costs more than it hides
+  // (up to 1.9x in the batch timings), so there the threshold starts at 128.
+  const double n = (double)nrows, mean = (double)(ptr[nrows] - ptr[0]) / n;
+  double var = 0.0;
+  long big32 = 0, big256 = 0;
+  for (long i = 0; i < nrows; i++) {
+    const int d = ptr[i + 1] - ptr[i];
+    if (d < 0) return fail(MPKX_EINVAL, "batch_default_config: decreasing ptr at row %ld", i);
+    var += (d - mean) * (d - mean);
+    big32 += d > 32;
+    big256 += d > 256;
+  }
+  const double cv = std::sqrt(var / n) / std::max(mean, 1e-9);
+  if (block_width) *block_width = nrhs < 16 ? nrhs : 16;
+  if (lanes_per_row) *lanes_per_row = cv > 0.5 ? 8 : mean <= 12.0 ? 2 : 4;
+  int t = big32 <= 0.3 * n ? 32 : big256 <= 0.3 * n ? 128 : 512;
+  if (nmat > 1 && cv > 0.5) t = std::max(t, 128);
+  if (long_row) *long_row = t;
+  return MPKX_OK;
+}
+
+int mpkx_batch_config(const mpkx_batch *b, int *block_width, int *lanes_per_row, int *long_row) {
+  clear_error();
+  if (!b) return fail(MPKX_EINVAL, "batch_config: null");
+  if (block_width) *block_width = b->B;
+  if (lanes_per_row) *lanes_per_row = b->L;
+  if (long_row) *long_row = b->T;
+  return MPKX_OK;
+}
+
 int mpkx_batch_configure(mpkx_batch *b, int block_width, int lanes_per_row, int long_row) {
   clear_error();
   if (!b) return fail(MPKX_EINVAL, "batch_configure: null");
@@ -335,7 +378,10 @@ int mpkx_batch_create(mpkx_batch **out, int nmat, const int *ns, const int *cons
     if (bad[m]) return fail(MPKX_EINVAL, "batch_create: matrix %d has %s", m, bad[m] == 1 ? "decreasing ptr" : "a column out of range");
   int rc;
   if ((rc = b->rp.upload(b->rp_host)) || (rc = b->ci.upload(ci)) || (rc = b->va.upload(va))) return rc;
-  if ((rc = mpkx_batch_configure(b, nrhs < 16 ? nrhs : 16, 4, 32))) return rc;
+  int B, L, T;
+  if ((rc = mpkx_batch_default_config(nmat, ntot, b->rp_host.data(), nrhs, &B, &L, &T)) ||
+      (rc = mpkx_batch_configure(b, B, L, T)))
+    return rc;
   *out = guard.release();
   return MPKX_OK;
 }

@@ -2,6 +2,8 @@
 synthetic generators, matrix features (closed forms), the FP32 batched
 restatement in the oracle (exact on integer data, stated error otherwise,
 documented summation order) and the FP64 MLP oracle (finite differences)."""
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -213,8 +215,53 @@ def test_config_space_and_targets():
     times = {c: 1.0 for c in CM.space(8)}
     times[(4, 2, 128)] = 0.5
     np.testing.assert_allclose(CM.soft_target(times), np.log2([4, 2, 128]), atol=1e-6)
-    assert CM.default_config(4) == (4, 4, 32)
-    assert all(CM.default_config(r) in CM.space(r) for r in (1, 2, 4, 8, 16))
+    f = np.zeros(24)
+    assert all(CM.default_config(r, f) in CM.space(r) for r in (1, 2, 4, 8, 16))
+
+
+@pytest.mark.parametrize("case", ["m5p", "cube", "banded20", "banded40", "chunglu", "wide", "empty_rows"])
+def test_default_config_rule(case):
+    """mpkx_batch_default_config (C, from the row pointer) and its mirror
+    over the features agree, and pick what the rule states for each kind of
+    row-length distribution."""
+    nrhs = 8
+    if case == "m5p":
+        rp, ci, _ = M.gen2("m5p", 64, -1.0)
+        want = (8, 2, 32)  # uniform short rows
+    elif case == "cube":
+        rp, ci, _ = M.gen2("cube", 12, -1.0)
+        want = (8, 2, 32)  # 7-point stencil
+    elif case == "banded20":
+        rp, ci, _ = M.gen_banded(20000, 100, 20, 1.0, 3)
+        want = (8, 4, 32)  # uniform, mean above 12
+    elif case == "banded40":
+        rp, ci, _ = M.gen_banded(20000, 100, 40, 1.0, 3)
+        want = (8, 4, 128)  # every row longer than 32
+    elif case == "chunglu":
+        rp, ci, _ = M.gen_chunglu(20000, 12.0, 2.5, 1.0, 5)
+        want = (8, 8, 32)  # power law: varied rows
+    elif case == "wide":
+        # 300 entries in every row
+        n = 1000
+        rp = np.arange(0, 300 * (n + 1), 300, dtype=np.int32)
+        ci = (np.arange(300 * n) % n).astype(np.int32)
+        want = (8, 4, 512)
+    else:
+        # every other row empty, the rest 3 entries: CV 1
+        n = 1000
+        lens = np.where(np.arange(n) % 2 == 0, 0, 3)
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        ci = (np.arange(rp[-1]) % n).astype(np.int32)
+        want = (8, 8, 32)
+    got = M.batch_default_config(rp, nrhs)
+    assert got == want
+    f = M.features(rp, ci, nrhs)
+    assert CM.default_config(nrhs, f) == got
+    assert M.batch_default_config(rp, 32)[0] == 16
+    # several matrices: varied rows start the long-row threshold at 128
+    many = M.batch_default_config(rp, nrhs, nmat=3)
+    assert many == (want[0], want[1], max(want[2], 128) if f[3] > 0.5 else want[2])
+    assert CM.default_config(nrhs, f, 3) == many
 
 
 def test_cost_model_roundtrip(tmp_path):
@@ -232,6 +279,50 @@ def test_suite_is_reproducible():
     assert {s["kind"] for s in a} == {"stencil", "banded", "chunglu"}
     rp, ci, _ = CM.make_matrix(a[1])
     assert len(rp) - 1 == a[1]["n"]
+
+
+def test_batch_suite_and_concat():
+    """Batch specs are reproducible and bounded (8M rows); the features of a
+    batch are those of its block-diagonal concatenation."""
+    a, b = CM.batch_suite(12, seed=5), CM.batch_suite(12, seed=5)
+    assert a == b
+    assert {s["sub"] for s in a} == {"stencil", "banded", "chunglu"}
+    assert all(1 <= len(s["ns"]) <= 512 and sum(s["ns"]) <= 8_000_000 for s in a)
+    mats = [M.gen2("m5p", 6, -1.0), M.gen_banded(50, 4, 5, 1.0, 2), M.gen_chunglu(80, 6.0, 2.5, 1.0, 3)]
+    ptr, col = CM.concat(mats)
+    want = sp.block_diag([sp.csr_matrix((v, c, p)) for p, c, v in mats], format="csr")
+    assert np.array_equal(ptr, want.indptr) and np.array_equal(col, want.indices)
+    small = dict(a[2], ns=[1200, 3000])
+    got = CM.make_matrix(small)
+    assert isinstance(got, list) and len(got) == 2
+    if small["sub"] != "stencil":
+        assert [len(m[0]) - 1 for m in got] == [1200, 3000]
+
+
+@pytest.mark.parametrize("name", ["eval_suite_seed11", "eval_batch_seed22"])
+def test_committed_model_on_fresh_sets(name):
+    """The committed cost model on the committed evaluation sets (suite and
+    batch seeds not used for training, timed exhaustively on the GPU), its
+    forward pass from the CPU FP64 MLP: the mean regret it records is
+    reproduced, and it is below the row-statistics default's and the best
+    fixed configuration's."""
+    root = os.path.join(os.path.dirname(M.__file__))
+    model = CM.CostModel.load(os.path.join(root, "cost_model.json"))
+    specs, feats, timings = CM.load_data(os.path.join(root, "tuning_data", name + ".npz"))
+    X = model.normalise(feats)
+    _, y, _ = O.mlp_loss_grad(X, np.zeros((len(X), 3)), model.params)
+    picks = [CM.nearest(p, s["nrhs"]) for p, s in zip(y, specs)]
+    rm = CM.regret(timings, picks)
+    rd = CM.regret(timings, [CM.default_config(s["nrhs"], f, CM.spec_nmat(s)) for s, f in zip(specs, feats)])
+    fixed = {int(r): tuple(c) for r, c in model.meta["evaluation"]["best_fixed_configs"].items()}
+    rf = CM.regret(timings, [fixed[s["nrhs"]] for s in specs])
+    key = {"eval_suite_seed11": "fresh_suite_seed11", "eval_batch_seed22": "fresh_batches_seed22"}[name]
+    rec = model.meta["evaluation"][key]
+    assert rec["n"] == len(specs)
+    assert abs(rm.mean() - rec["model_mean_regret"]) < 0.005
+    assert abs(rd.mean() - rec["default_mean_regret"]) < 1e-6
+    assert rm.mean() < rd.mean() < rf.mean()
+    assert rm.mean() < 1.1
 
 
 def test_oracle_cg_against_direct_solve():
