@@ -109,6 +109,48 @@ def cpu_baseline_reference(M, rp, ci, va, x, N, k, gold_check):
             "results_bitwise_equal_gpu": ok}
 
 
+def cpu_threaded_kxspmv(rp, ci, va, x, k, gold, reps=5):
+    """The oracle's k x CSR SpMV with each power's rows split over the host
+    threads (min(cores, 16)), on the full workload: the plain multi-core
+    CPU path beside the reference's MPI exec.  Min of `reps`."""
+    from tests import oracle_api as O
+    threads = min(os.cpu_count() or 1, 16)
+    best, used, ok = None, 0, False
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        vv, used = O.spmv_k_par(rp, ci, va, x, k, threads)
+        d = time.perf_counter() - t0
+        best = d if best is None else min(best, d)
+        ok = bool(np.array_equal(vv, gold))
+    n = len(rp) - 1
+    B = algorithmic_bytes(n, len(ci))
+    return {"value": round(k * B / best / 1e9, 3), "unit": "GB/s", "threads": used,
+            "ms_per_mpk": round(best * 1e3, 3),
+            "sample": f"oracle k x CSR SpMV (mpk2/verify_val.c order) on {used} threads, rows split per "
+                      f"power, full workload, min of {reps} reps",
+            "results_bitwise_equal_gpu": ok}
+
+
+def copy_peak(M, torch, dev, stream, reps=10):
+    """Attainable HBM bandwidth: k_dcopy over 512 MiB (read + write), events
+    on the launch stream."""
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        M.dcopy(dst, src, n, stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        M.dcopy(dst, src, n, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return {"value": round(2 * 8 * n / (ms * 1e-3) / 1e9, 2), "unit": "GB/s", "kernel": "k_dcopy, 512 MiB",
+            "ms": round(ms, 5), "pct_of_8tbs": round(100 * 2 * 8 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2)}
+
+
 def measured_traffic(N, k, s, blocks, codec):
     """HBM bytes per launch of the dominant fused round, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this configuration and value
@@ -578,6 +620,11 @@ def main():
         torch.cuda.synchronize()
         del big, big2
 
+    try:
+        peak = copy_peak(M, torch, dev, stream)
+    except Exception as e:
+        peak = {"error": str(e)}
+
     B = algorithmic_bytes(n, nnz)
     gbs_fused = k * B / (ms_fused * 1e-3) / 1e9
     gbs_plain = k * B / (ms_plain * 1e-3) / 1e9
@@ -606,8 +653,12 @@ def main():
                    "partitions": int(np.prod(blocks)), "phases": nround,
                    "tile_blocks": blocks, "parallelism": "single GPU"},
         "pct_hbm_roofline": round(100 * gbs_fused / HBM_PEAK_GBS, 2),
+        "measured_copy_peak": peak,
+        "pct_measured_copy_peak": round(100 * gbs_fused / peak["value"], 2) if "value" in peak else None,
         "baseline_kxspmv": {"value": round(gbs_plain, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 5),
-                            "pct_hbm_roofline": round(100 * gbs_plain / HBM_PEAK_GBS, 2)},
+                            "pct_hbm_roofline": round(100 * gbs_plain / HBM_PEAK_GBS, 2),
+                            "pct_measured_copy_peak": round(100 * gbs_plain / peak["value"], 2)
+                            if "value" in peak else None},
         "speedup_vs_kxspmv": round(ms_plain / ms_fused, 3),
         "kxspmv_rocsparse": None if ms_rs is None else {
             "ms_per_step": round(ms_rs, 5), "value": round(k * B / (ms_rs * 1e-3) / 1e9, 2), "unit": "GB/s"},
@@ -674,6 +725,10 @@ def main():
             print(f"reference CPU baseline unavailable: {e}", file=sys.stderr)
         if cb is None:
             cb = cpu_baseline_port(rp, ci, va, x, k, args.cpu_seconds)
+        try:
+            cb["threaded_kxspmv"] = cpu_threaded_kxspmv(rp, ci, va, x, k, host_vv)
+        except Exception as e:
+            cb["threaded_kxspmv"] = {"error": str(e)}
         out["cpu_baseline"] = cb
     print(json.dumps(out))
 

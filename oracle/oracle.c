@@ -24,6 +24,8 @@
  *   orc_exec_bufs       bitwise equal to reference mpi2_mpkexecbufs_val
  *                       results (harness with seeded x) and to orc_spmv_k
  *   orc_spmv_k          bitwise equal to reference verify_val gold_result.bin
+ *   orc_spmv_k_par      bitwise equal to orc_spmv_k (rows split over
+ *                       threads, each row summed in CSR order)
  *   orc_pd_exec         the final power equals orc_spmv_k to <= 1e-13
  *                       normwise (partial sums reorder the additions)
  *                       and passes the reference cosine check
@@ -52,6 +54,7 @@
 #include <string.h>
 #include <limits.h>
 #include <math.h>
+#include <pthread.h>
 
 #define ORC_API __attribute__((visibility("default")))
 
@@ -844,6 +847,76 @@ ORC_API void orc_spmv_k(int n, const int *ptr, const int *col, const double *val
       out[i] = s;
     }
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* The same k SpMVs on up to nthreads host threads (the multi-core CPU */
+/* baseline): of nt threads that started, thread t takes rows          */
+/* [n*t/nt, n*(t+1)/nt) of every power, a barrier separates the        */
+/* powers.  Each row is summed in CSR order, so vv is bitwise equal to */
+/* orc_spmv_k's.  Returns the number of threads used.                  */
+/* ------------------------------------------------------------------ */
+struct spmv_shared {
+  int n, k, nt, go;
+  const int *ptr, *col;
+  const double *val;
+  double *vv;
+  pthread_mutex_t mu;
+  pthread_cond_t cv;
+  pthread_barrier_t bar;
+};
+
+struct spmv_part {
+  struct spmv_shared *S;
+  int t;
+};
+
+static void *spmv_rows(void *arg) {
+  const struct spmv_part *J = arg;
+  struct spmv_shared *S = J->S;
+  pthread_mutex_lock(&S->mu);  /* start gate: nt and the barrier are set */
+  while (!S->go) pthread_cond_wait(&S->cv, &S->mu);
+  pthread_mutex_unlock(&S->mu);
+  const long i0 = (long)S->n * J->t / S->nt, i1 = (long)S->n * (J->t + 1) / S->nt;
+  for (int l = 0; l < S->k; l++) {
+    const double *in = S->vv + (long)l * S->n;
+    double *out = S->vv + (long)(l + 1) * S->n;
+    for (long i = i0; i < i1; i++) {
+      double s = 0.0;
+      for (int j = S->ptr[i]; j < S->ptr[i + 1]; j++) s += S->val[j] * in[S->col[j]];
+      out[i] = s;
+    }
+    pthread_barrier_wait(&S->bar);
+  }
+  return NULL;
+}
+
+ORC_API int orc_spmv_k_par(int n, const int *ptr, const int *col, const double *val,
+                           const double *x, int k, double *vv, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  memcpy(vv, x, sizeof(double) * n);
+  struct spmv_shared S = {.n = n, .k = k, .nt = 1, .go = 0, .ptr = ptr, .col = col, .val = val, .vv = vv};
+  pthread_mutex_init(&S.mu, NULL);
+  pthread_cond_init(&S.cv, NULL);
+  struct spmv_part *jobs = malloc(sizeof(struct spmv_part) * nthreads);
+  pthread_t *th = malloc(sizeof(pthread_t) * nthreads);
+  for (int t = 0; t < nthreads; t++) jobs[t] = (struct spmv_part){&S, t};
+  int nt = 1;  /* thread 0 is the caller */
+  while (nt < nthreads && pthread_create(&th[nt], NULL, spmv_rows, &jobs[nt]) == 0) nt++;
+  pthread_barrier_init(&S.bar, NULL, (unsigned)nt);
+  pthread_mutex_lock(&S.mu);
+  S.nt = nt;
+  S.go = 1;
+  pthread_cond_broadcast(&S.cv);
+  pthread_mutex_unlock(&S.mu);
+  spmv_rows(&jobs[0]);
+  for (int t = 1; t < nt; t++) pthread_join(th[t], NULL);
+  pthread_barrier_destroy(&S.bar);
+  pthread_cond_destroy(&S.cv);
+  pthread_mutex_destroy(&S.mu);
+  free(th);
+  free(jobs);
+  return nt;
 }
 
 /* ------------------------------------------------------------------ */

@@ -38,6 +38,8 @@ def lib():
         L.orc_free.argtypes = [P]
         L.orc_exec_bufs.argtypes = [C.c_int, P, P, P]
         L.orc_spmv_k.argtypes = [C.c_int, P, P, P, P, C.c_int, P]
+        L.orc_spmv_k_par.argtypes = [C.c_int, P, P, P, P, C.c_int, P, C.c_int]
+        L.orc_spmv_k_par.restype = C.c_int
         L.orc_prep_tasks.argtypes = [C.c_int, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P, P]
         L.orc_prep_tasks.restype = C.c_long
         L.orc_vexec_tasks.argtypes = [C.c_int, P, P, P, C.c_long, P, P]
@@ -145,6 +147,14 @@ def spmv_k(rp, ci, va, x, k):
     vv = np.zeros(n * (k + 1), np.float64)
     lib().orc_spmv_k(n, ptr(rp), ptr(ci), ptr(va), ptr(np.ascontiguousarray(x)), k, ptr(vv))
     return vv
+
+
+def spmv_k_par(rp, ci, va, x, k, nthreads):
+    """orc_spmv_k on host threads; returns (vv, threads used)."""
+    n = len(rp) - 1
+    vv = np.zeros(n * (k + 1), np.float64)
+    used = lib().orc_spmv_k_par(n, ptr(rp), ptr(ci), ptr(va), ptr(np.ascontiguousarray(x)), k, ptr(vv), nthreads)
+    return vv, used
 
 
 def prep_tasks(rp, ci, nlevel, nphase, parts, levs, sl):

@@ -83,6 +83,12 @@ def test_exec_bitwise_equal_reference(case):
     assert O.sha(gold.tobytes()) == d["results_seeded.bin"]
     ones = O.spmv_k(rp, ci, va, np.ones(d["n"]), d["nlevel"])
     assert O.sha(ones.tobytes()) == d["gold_result.bin"]
+    # the multi-threaded CPU baseline: the same bits for any thread count,
+    # including more threads than rows on the small cases
+    for nt in (1, 3, 8) + ((d["n"] + 5,) if d["n"] <= 256 else ()):
+        par, used = O.spmv_k_par(rp, ci, va, x, d["nlevel"], nt)
+        assert used == nt
+        assert O.sha(par.tobytes()) == d["results_seeded.bin"]
 
 
 def test_uniform_matches_python_restatement():
