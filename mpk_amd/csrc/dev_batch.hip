@@ -14,7 +14,8 @@
 //               16-byte vectors); the L partial sums meet in a butterfly of
 //               lane shuffles.  One (col, val) load serves B products.
 //   long rows:  (nnz > T) a 256-lane workgroup per row; the lanes stride the
-//               row and reduce in LDS.
+//               row and reduce in LDS.  Both kinds run in one launch per
+//               power and column block, the long rows' workgroups first.
 //   passes:     R/B passes over the matrix per power when B < R.
 //   XCD order:  workgroup i takes row block xcd_remap(i), so each of the 8
 //               XCDs walks a contiguous range of rows (whole matrices) and
@@ -104,17 +105,16 @@ __device__ __forceinline__ void accumulate(int j0, int e, const int *__restrict_
       for (int b = 0; b < B; b++) acc[b] = fmaf(aj[u], xv[u][b], acc[b]);
 }
 
-// L lanes per row; lane l sums entries rp[row] + l, + L, ... for B
-// right-hand sides held in registers; the L partial sums meet in a
-// butterfly of lane shuffles.
+// Group rows: L lanes per row; lane l sums entries rp[row] + l, + L, ...
+// for B right-hand sides held in registers; the L partial sums meet in a
+// butterfly of lane shuffles.  blk is this workgroup's block of L-lane slots.
 template <int B, int L>
-__global__ __launch_bounds__(BT) void k_spmm_group(const int *__restrict__ rows, int nrows,
-                                                   const int *__restrict__ rp, const int *__restrict__ ci,
-                                                   const float *__restrict__ va, const float *__restrict__ x,
-                                                   float *__restrict__ y, int ldv, int c0) {
+__device__ __forceinline__ void group_rows(int blk, const int *__restrict__ rows, int nrows,
+                                           const int *__restrict__ rp, const int *__restrict__ ci,
+                                           const float *__restrict__ va, const float *__restrict__ x,
+                                           float *__restrict__ y, int ldv, int c0) {
   static_assert(L <= 64 && (L & (L - 1)) == 0, "row group must fit a wavefront");
-  // consecutive rows (one matrix, one x segment) stay on one XCD's L2
-  const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * BT + threadIdx.x;
+  const long g = (long)blk * BT + threadIdx.x;
   const long slot = g / L;
   const int l = (int)(g % L);
   const bool active = slot < nrows;
@@ -134,15 +134,12 @@ __global__ __launch_bounds__(BT) void k_spmm_group(const int *__restrict__ rows,
   if (active && l == 0) store_rhs<B>(y + (size_t)row * ldv + c0, acc);
 }
 
-// Rows longer than T: one 256-lane workgroup per row, lane t sums entries
-// t, t + 256, ...; the partial sums are combined by a pairwise tree in LDS.
+// A long row on the whole workgroup: lane t sums entries t, t + 256, ...;
+// the partial sums are combined by a pairwise tree in LDS.
 template <int B>
-__global__ __launch_bounds__(BT) void k_spmm_long(const int *__restrict__ rows, const int *__restrict__ rp,
-                                                  const int *__restrict__ ci, const float *__restrict__ va,
-                                                  const float *__restrict__ x, float *__restrict__ y, int ldv,
-                                                  int c0) {
-  __shared__ float red[B][BT];
-  const int row = rows[xcd_remap(blockIdx.x, gridDim.x)];
+__device__ __forceinline__ void long_row(int row, float (&red)[B][BT], const int *__restrict__ rp,
+                                         const int *__restrict__ ci, const float *__restrict__ va,
+                                         const float *__restrict__ x, float *__restrict__ y, int ldv, int c0) {
   const int t = threadIdx.x;
   float acc[B];
 #pragma unroll
@@ -162,6 +159,30 @@ __global__ __launch_bounds__(BT) void k_spmm_long(const int *__restrict__ rows, 
   if (t < B) y[(size_t)row * ldv + c0 + t] = red[t][0];
 }
 
+// One power (B right-hand sides from column c0) in one launch: workgroups
+// [0, nlong_pad) take the long rows, one each, the rest the group rows.  The
+// long rows are dispatched first so that they start early and the short-row
+// blocks fill in behind them (a separate long-row launch after the group
+// launch left its 21504 rows as a 140 us tail per power on a batch of 256
+// Chung-Lu matrices).  nlong_pad is nlong rounded up to a multiple of the 8
+// XCDs, so both regions start on XCD 0 and xcd_remap keeps consecutive rows
+// (one matrix, one x segment) on one XCD's L2 in each.
+template <int B, int L>
+__global__ __launch_bounds__(BT) void k_spmm_level(const int *__restrict__ lrows, int nlong, int nlong_pad,
+                                                   const int *__restrict__ srows, int nshort,
+                                                   const int *__restrict__ rp, const int *__restrict__ ci,
+                                                   const float *__restrict__ va, const float *__restrict__ x,
+                                                   float *__restrict__ y, int ldv, int c0) {
+  __shared__ float red[B][BT];
+  const int i = blockIdx.x;
+  if (i < nlong_pad) {
+    const int q = xcd_remap(i, nlong_pad);
+    if (q < nlong) long_row<B>(lrows[q], red, rp, ci, va, x, y, ldv, c0);
+    return;
+  }
+  group_rows<B, L>(xcd_remap(i - nlong_pad, (int)gridDim.x - nlong_pad), srows, nshort, rp, ci, va, x, y, ldv, c0);
+}
+
 struct LevelArgs {
   const int *srows, *lrows, *rp, *ci;
   const float *va;
@@ -169,32 +190,25 @@ struct LevelArgs {
 };
 
 template <int B, int L>
-void launch_group(const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
-  long threads = (long)a.nshort * L;
-  hipLaunchKernelGGL((k_spmm_group<B, L>), dim3((unsigned)((threads + BT - 1) / BT)), dim3(BT), 0, st, a.srows,
-                     a.nshort, a.rp, a.ci, a.va, x, y, a.ldv, c0);
-}
-
-template <int B>
-bool launch_group_l(int L, const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
-  switch (L) {
-    case 1: launch_group<B, 1>(a, x, y, c0, st); return true;
-    case 2: launch_group<B, 2>(a, x, y, c0, st); return true;
-    case 4: launch_group<B, 4>(a, x, y, c0, st); return true;
-    case 8: launch_group<B, 8>(a, x, y, c0, st); return true;
-    case 16: launch_group<B, 16>(a, x, y, c0, st); return true;
-    case 32: launch_group<B, 32>(a, x, y, c0, st); return true;
-    case 64: launch_group<B, 64>(a, x, y, c0, st); return true;
-  }
-  return false;
+void launch_level_bl(const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
+  const int nlong_pad = (a.nlong + 7) / 8 * 8;
+  const long gblocks = ((long)a.nshort * L + BT - 1) / BT;
+  hipLaunchKernelGGL((k_spmm_level<B, L>), dim3((unsigned)(nlong_pad + gblocks)), dim3(BT), 0, st, a.lrows,
+                     a.nlong, nlong_pad, a.srows, a.nshort, a.rp, a.ci, a.va, x, y, a.ldv, c0);
 }
 
 template <int B>
 bool launch_level_b(int L, const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
-  if (a.nshort > 0 && !launch_group_l<B>(L, a, x, y, c0, st)) return false;
-  if (a.nlong > 0)
-    hipLaunchKernelGGL((k_spmm_long<B>), dim3(a.nlong), dim3(BT), 0, st, a.lrows, a.rp, a.ci, a.va, x, y, a.ldv, c0);
-  return true;
+  switch (L) {
+    case 1: launch_level_bl<B, 1>(a, x, y, c0, st); return true;
+    case 2: launch_level_bl<B, 2>(a, x, y, c0, st); return true;
+    case 4: launch_level_bl<B, 4>(a, x, y, c0, st); return true;
+    case 8: launch_level_bl<B, 8>(a, x, y, c0, st); return true;
+    case 16: launch_level_bl<B, 16>(a, x, y, c0, st); return true;
+    case 32: launch_level_bl<B, 32>(a, x, y, c0, st); return true;
+    case 64: launch_level_bl<B, 64>(a, x, y, c0, st); return true;
+  }
+  return false;
 }
 
 bool launch_level(int B, int L, const LevelArgs &a, const float *x, float *y, int c0, hipStream_t st) {
@@ -406,7 +420,7 @@ int mpkx_batch_stats(const mpkx_batch *b, long *rows, long *nnz, long *long_rows
   if (rows) *rows = b->ntot;
   if (nnz) *nnz = b->nnz;
   if (long_rows) *long_rows = b->nlong;
-  if (launches) *launches = b->nlevel * (b->nrhs / b->B) * ((b->nshort > 0) + (b->nlong > 0));
+  if (launches) *launches = b->nlevel * (b->nrhs / b->B);
   return MPKX_OK;
 }
 
