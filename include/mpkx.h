@@ -220,14 +220,21 @@ typedef struct mpkx_plan mpkx_plan;
  * each position (stencils with constant coefficients) the plan keeps one
  * value per position instead of mval_buf (same products, same sums). */
 int mpkx_plan_create(mpkx_plan **plan, mpkx_buffers *const *bufs, int npart);
-/* The same with options: MPKX_PLAN_EXPLICIT keeps one value per entry. */
-enum { MPKX_PLAN_EXPLICIT = 1 };
+/* The same with options: MPKX_PLAN_EXPLICIT keeps one value per entry;
+ * MPKX_PLAN_WIDE keeps 64-bit row pointers, 32-bit columns and one output
+ * slot per row for every launch (otherwise used only by the launches the
+ * compact form does not fit: 32-bit row pointers, 16-bit column offsets
+ * per workgroup, one output base per workgroup).  Results are the same
+ * bits either way. */
+enum { MPKX_PLAN_EXPLICIT = 1, MPKX_PLAN_WIDE = 2 };
 int mpkx_plan_create_opts(mpkx_plan **plan, mpkx_buffers *const *bufs,
                           int npart, int flags);
 /* positional: 1 when the plan keeps positional values; matrix_bytes:
- * device bytes of its row pointers, columns and values. */
+ * device bytes of its row structure, columns and values; compact: how
+ * many of its (phase, level) launches use the compact structure (the
+ * others, and every launch with MPKX_PLAN_WIDE, use the wide one). */
 int mpkx_plan_info(const mpkx_plan *plan, int *positional,
-                   long *matrix_bytes);
+                   long *matrix_bytes, int *compact);
 void mpkx_plan_destroy(mpkx_plan *plan);
 /* Host drop-in: takes each bufs[p]->vv_buf[0 .. rcount[0]) as the input
  * (as the reference main loads it), runs every phase, and writes every

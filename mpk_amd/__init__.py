@@ -98,7 +98,7 @@ def lib():
         "mpkx_write_results": ([C.c_char_p, C.c_int, C.c_int, P], C.c_int),
         "mpkx_execbufs_dir": ([C.c_char_p, P, P], C.c_int),
         "mpkx_mpkrun_dir": ([C.c_char_p, P, P], C.c_int),
-        "mpkx_plan_info": ([P, P, P], C.c_int),
+        "mpkx_plan_info": ([P, P, P, P], C.c_int),
         "mpkx_plan_destroy": ([P], None),
         "mpkx_exec_bufs": ([P, P], C.c_int),
         "mpkx_exec": ([P, P, P, P, C.c_int], C.c_int),
@@ -534,6 +534,7 @@ class Plan:
     """GPU exec of reference buffers (drop-in for mpk_exec_bufs_val on P ranks)."""
 
     EXPLICIT = 1  # keep one value per entry (no positional values)
+    WIDE = 2  # 64-bit row pointers, 32-bit columns, an output slot per row (no compact structure)
 
     def __init__(self, bufset, flags=0):
         self._h = C.c_void_p()
@@ -548,10 +549,10 @@ class Plan:
     def stats(self):
         r, z, g, l = C.c_long(), C.c_long(), C.c_long(), C.c_int()
         _check(lib().mpkx_plan_stats(self._h, C.byref(r), C.byref(z), C.byref(g), C.byref(l)))
-        pos, mb = C.c_int(), C.c_long()
-        _check(lib().mpkx_plan_info(self._h, C.byref(pos), C.byref(mb)))
+        pos, mb, cp = C.c_int(), C.c_long(), C.c_int()
+        _check(lib().mpkx_plan_info(self._h, C.byref(pos), C.byref(mb), C.byref(cp)))
         return {"rows": r.value, "nnz": z.value, "ghosts": g.value, "launches": l.value,
-                "positional": bool(pos.value), "matrix_bytes": mb.value}
+                "positional": bool(pos.value), "compact_launches": cp.value, "matrix_bytes": mb.value}
 
     def __del__(self, _release=_release):
         if getattr(self, "_h", None):

@@ -95,6 +95,68 @@ __global__ __launch_bounds__(CS_THREADS) void k_csr_stream(
   }
 }
 
+// Compact form of the same kernel for the reference's buffers, whose
+// matrix is stored once per level and so streams from HBM at every level:
+// 32-bit row pointers, columns as 16-bit offsets from the workgroup's
+// anchor slot (its smallest input), and the outputs of a workgroup in
+// consecutive slots from bout[b] (a workgroup never spans two partitions,
+// and a partition's rows of one level fill consecutive mbuf slots).  The
+// products, their order and the sums are those of k_csr_stream.
+template <bool POS>
+__global__ __launch_bounds__(CS_THREADS) void k_csr_compact(
+    const int *__restrict__ rp, const uint16_t *__restrict__ ci, const double *__restrict__ va,
+    const double *x, double *y, const int *__restrict__ banchor, const int *__restrict__ bout,
+    const int *__restrict__ blk, int blk0) {
+  __shared__ double prod[CS_CAP];
+  __shared__ double pv[POS ? POS_MAX : 1];
+  const int b = blk0 + blockIdx.x;
+  const int r0 = blk[b], r1 = blk[b + 1];
+  const int j0 = rp[r0], j1 = rp[r1];
+  const int len = j1 - j0;
+  const double *xa = x + banchor[b];
+  const int o0 = bout[b];
+  if (len <= CS_CAP) {
+    if constexpr (POS) {
+      if (threadIdx.x < POS_MAX) pv[threadIdx.x] = va[threadIdx.x];
+    }
+    for (int t0 = threadIdx.x; t0 < len; t0 += CS_BATCH * CS_THREADS) {
+      int cj[CS_BATCH];
+      double av[CS_BATCH], xv[CS_BATCH];
+#pragma unroll
+      for (int u = 0; u < CS_BATCH; u++) {
+        const int t = t0 + u * CS_THREADS;
+        cj[u] = 0;
+        av[u] = 0.0;
+        if (t < len) {
+          cj[u] = ci[j0 + t];
+          if constexpr (!POS) av[u] = va[j0 + t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CS_BATCH; u++) {
+        xv[u] = 0.0;
+        if (t0 + u * CS_THREADS < len) xv[u] = xa[cj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < CS_BATCH; u++) {
+        const int t = t0 + u * CS_THREADS;
+        if (t < len) prod[t] = POS ? xv[u] : __dmul_rn(av[u], xv[u]);
+      }
+    }
+    __syncthreads();
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += CS_THREADS) {
+      double s = 0.0;
+      const int jr = rp[r], e = rp[r + 1];
+      for (int j = jr; j < e; j++) s = __dadd_rn(s, POS ? __dmul_rn(pv[j - jr], prod[j - j0]) : prod[j - j0]);
+      y[o0 + (r - r0)] = s;
+    }
+  } else if (threadIdx.x == 0) {  // a single row longer than the LDS stage
+    double s = 0.0;
+    for (int j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(POS ? va[j - j0] : va[j], xa[ci[j]]));
+    y[o0] = s;
+  }
+}
+
 __global__ void k_copy_pairs(double *flat, const int *__restrict__ dst, const int *__restrict__ src,
                              long count) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,7 +196,7 @@ void make_blocks(const std::vector<PtrT> &rp, int r0, int r1, std::vector<int> &
     PtrT start = rp[r];
     int e = r + 1;
     if (rp[e] - start <= CS_CAP)
-      while (e < r1 && rp[e + 1] - start <= CS_CAP && e - r < 4 * CS_THREADS) e++;
+      while (e < r1 && rp[e + 1] - start <= CS_CAP && e - r < CS_THREADS) e++;
     r = e;
   }
 }
@@ -154,6 +216,7 @@ struct mpkx_plan {
   // per (phase, level) launch groups
   struct Group {
     int ph, level, blk0, nblk;
+    bool compact;  // launched as k_csr_compact
   };
   std::vector<Group> groups;
   std::vector<long> gather_begin;  // per phase: range in gather pairs
@@ -162,6 +225,11 @@ struct mpkx_plan {
   DevArray<int> ci, out_row, blk, gdst, gsrc, xdst, xsrc, csrc;
   DevArray<double> va, vv, stage;  // va: one value per entry, or the positional values
   bool positional = false;
+  // compact structure (k_csr_compact) of the groups that fit it; rp, ci and
+  // out_row are kept only when some group does not
+  int compact_groups = 0;
+  DevArray<int> rp32, banchor, bout;
+  DevArray<uint16_t> ci16;
   long rows = 0, nnz = 0, ghosts = 0;
   GraphCache graph;
 };
@@ -203,6 +271,7 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
   std::vector<long> rp(1, 0);
   std::vector<int> ci, outr, blk;
   std::vector<double> va;
+  int gblk = 0;  // workgroups of the groups so far
   try {
     for (int ph = 0; ph <= nph; ph++) {
       // level ranges of every partition's mbuf in this phase
@@ -229,6 +298,7 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
           const long *mc = b->mcol_buf + b->mcol_offsets[ph];
           const double *mv = b->mval_buf + b->mcol_offsets[ph];
           int &m = cursor[p];
+          const int rp0 = (int)rp.size() - 1;
           for (; m < b->mcount[ph] && im[m] / n == l; m++) {
             for (long j = mp[m]; j < mp[m + 1]; j++) {
               if (mc[j] < 0 || mc[j] >= b->buf_count)
@@ -239,12 +309,14 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
             rp.push_back((long)ci.size());
             outr.push_back((int)(pl->base[p] + b->mbuf_offsets[ph] + m));
           }
+          // workgroups never span two partitions (consecutive output slots)
+          make_blocks(rp, rp0, (int)rp.size() - 1, blk);
         }
         int r1 = (int)rp.size() - 1;
         if (r1 == r0) continue;
-        int blk0 = (int)blk.size();
-        make_blocks(rp, r0, r1, blk);
-        pl->groups.push_back({ph, l, blk0, (int)blk.size() - blk0});
+        int blk0 = gblk;
+        gblk = (int)blk.size();
+        pl->groups.push_back({ph, l, blk0, gblk - blk0, false});
       }
     }
     blk.push_back((int)rp.size() - 1);
@@ -337,7 +409,52 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
     if (csrc[v] < 0)
       return fail(MPKX_ESTRUCT, "plan_create: level %zu row %zu is computed by no partition", v / n, v % n);
 
+  // Compact structure for a (phase, level) group when every workgroup's
+  // outputs are consecutive and its inputs lie within 65536 slots of its
+  // smallest one.  From phase 1 on a row may read both the previous phase's
+  // rows and the received ghosts, far apart in the buffer: such groups keep
+  // the wide structure.
+  std::vector<int> rp32, banchor, bout;
+  std::vector<uint16_t> ci16;
+  const long nblk = (long)blk.size() - 1;
+  if (!(flags & MPKX_PLAN_WIDE) && pl->nnz < INT32_MAX) {
+    try {
+      banchor.assign(nblk, 0);
+      bout.assign(nblk, 0);
+      ci16.assign(ci.size(), 0);
+      for (auto &g : pl->groups) {
+        bool ok = true;
+        for (long q = g.blk0; q < g.blk0 + g.nblk && ok; q++) {
+          const int r0 = blk[q], r1 = blk[q + 1];
+          for (int r = r0 + 1; r < r1 && ok; r++) ok = outr[r] == outr[r0] + (r - r0);
+          int lo = INT32_MAX, hi = -1;
+          for (long j = rp[r0]; j < rp[r1]; j++) lo = std::min(lo, ci[j]), hi = std::max(hi, ci[j]);
+          if (hi >= 0 && hi - lo > 65535) ok = false;
+          banchor[q] = hi >= 0 ? lo : 0;
+          bout[q] = outr[r0];
+        }
+        g.compact = ok;
+        if (!ok) continue;
+        pl->compact_groups++;
+        for (long q = g.blk0; q < g.blk0 + g.nblk; q++)
+          for (long j = rp[blk[q]]; j < rp[blk[q + 1]]; j++) ci16[j] = (uint16_t)(ci[j] - banchor[q]);
+      }
+      if (pl->compact_groups > 0) rp32.assign(rp.begin(), rp.end());
+    } catch (const std::bad_alloc &) {
+      return fail(MPKX_ENOMEM, "plan_create: host allocation");
+    }
+  }
   int rc;
+  if (pl->compact_groups > 0) {
+    if ((rc = pl->rp32.upload(rp32)) || (rc = pl->ci16.upload(ci16)) || (rc = pl->banchor.upload(banchor)) ||
+        (rc = pl->bout.upload(bout)))
+      return rc;
+    if (pl->compact_groups == (int)pl->groups.size()) {  // the wide structure is not needed
+      rp.clear();
+      ci.clear();
+      outr.clear();
+    }
+  }
   if ((rc = pl->rp.upload(rp)) || (rc = pl->ci.upload(ci)) || (rc = pl->va.upload(va)) ||
       (rc = pl->out_row.upload(outr)) || (rc = pl->blk.upload(blk)) || (rc = pl->gdst.upload(gdst)) ||
       (rc = pl->gsrc.upload(gsrc)) || (rc = pl->xdst.upload(xdst)) || (rc = pl->xsrc.upload(xsrc)) ||
@@ -359,12 +476,15 @@ int mpkx_dcopy(double *dst, const double *src, long n, void *stream) {
   return MPKX_OK;
 }
 
-int mpkx_plan_info(const mpkx_plan *pl, int *positional, long *matrix_bytes) {
+int mpkx_plan_info(const mpkx_plan *pl, int *positional, long *matrix_bytes, int *compact) {
   clear_error();
   if (!pl) return fail(MPKX_EINVAL, "plan_info: null plan");
   if (positional) *positional = pl->positional ? 1 : 0;
+  if (compact) *compact = pl->compact_groups;
   if (matrix_bytes)
-    *matrix_bytes = (long)(pl->rp.count * sizeof(long) + pl->ci.count * sizeof(int) + pl->va.count * sizeof(double));
+    *matrix_bytes = (long)(pl->rp.count * sizeof(long) + pl->ci.count * sizeof(int) + pl->va.count * sizeof(double) +
+                           pl->out_row.count * sizeof(int) + pl->rp32.count * sizeof(int) +
+                           pl->ci16.count * sizeof(uint16_t) + (pl->banchor.count + pl->bout.count) * sizeof(int));
   return MPKX_OK;
 }
 
@@ -405,7 +525,13 @@ static int run_phases(mpkx_plan *pl, hipStream_t st) {
     }
     for (; gi < pl->groups.size() && pl->groups[gi].ph == ph; gi++) {
       const auto &g = pl->groups[gi];
-      if (pl->positional)
+      if (g.compact && pl->positional)
+        hipLaunchKernelGGL((k_csr_compact<true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp32.p, pl->ci16.p,
+                           pl->va.p, pl->vv.p, pl->vv.p, pl->banchor.p, pl->bout.p, pl->blk.p, g.blk0);
+      else if (g.compact)
+        hipLaunchKernelGGL((k_csr_compact<false>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp32.p, pl->ci16.p,
+                           pl->va.p, pl->vv.p, pl->vv.p, pl->banchor.p, pl->bout.p, pl->blk.p, g.blk0);
+      else if (pl->positional)
         hipLaunchKernelGGL((k_csr_stream<long, true, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
                            pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
       else

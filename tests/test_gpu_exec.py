@@ -40,9 +40,12 @@ def test_exact_plan_bitwise_reference(case):
     # the positional plan (when the values are constant per row position,
     # as for the cube's diagonal-first rows) and the plan with one value per
     # entry both give the reference's bits
-    for flags, positional in ((0, _positional(rp, va)), (M.Plan.EXPLICIT, False)):
+    # and so do the compact structure (16-bit column offsets) and the wide one
+    for flags, positional in ((M.Plan.WIDE, _positional(rp, va)), (M.Plan.EXPLICIT | M.Plan.WIDE, False),
+                              (M.Plan.EXPLICIT, False), (0, _positional(rp, va))):
         plan = M.Plan(bufs, flags)
         assert plan.stats()["positional"] == positional
+        assert (plan.stats()["compact_launches"] > 0) == (not flags & M.Plan.WIDE)
         vv = zeros_dev(d["n"] * (d["nlevel"] + 1))
         plan.exec(to_dev(x), vv)
         out = to_host(vv)
@@ -386,7 +389,31 @@ def test_exact_plan_headline_size_bitwise_oracle():
     gold = O.spmv_k(rp, ci, va, x, k)
     parts = [M.partition_grid([N] * 3, [4, 4, 4], [16 * ph] * 3) for ph in range(nphase)]
     _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
+    for flags in (0, M.Plan.WIDE):
+        plan = M.Plan(bufs, flags)
+        # phase 0 fits the compact structure; later phases read both earlier
+        # rows and received ghosts, which may lie too far apart
+        assert (plan.stats()["compact_launches"] > 0) == (flags == 0)
+        vv = zeros_dev(n * (k + 1))
+        plan.exec(to_dev(x), vv)
+        assert np.array_equal(to_host(vv), gold), flags
+        del plan
+
+
+def test_exact_plan_wide_fallback():
+    """A banded matrix whose rows read slots 90000 apart (beyond 16-bit
+    offsets) keeps the wide structure, with the same bits."""
+    say("exact plan, banded 200k (bandwidth 90000), 2 partitions")
+    rp, ci, va = M.gen_banded(200000, 90000, 5, 1.0, 41)
+    n = len(rp) - 1
+    k = 2
+    part = (np.arange(n) * 2 // n).astype(np.int32)
+    _, _, bufs, _ = M.prepare(rp, ci, va, k, 0, [part])
+    x = M.uniform_pm1(0x5EED, n)
+    gold = O.spmv_k(rp, ci, va, x, k)
     plan = M.Plan(bufs)
+    st = plan.stats()
+    assert st["compact_launches"] == 0 and not st["positional"]
     vv = zeros_dev(n * (k + 1))
     plan.exec(to_dev(x), vv)
     assert np.array_equal(to_host(vv), gold)
@@ -407,12 +434,14 @@ def test_exact_plan_cube64_against_oracle(values):
     for nphase in (0, 1):
         parts = [M.partition_grid([64] * 3, [2, 2, 2], [16 * ph] * 3) for ph in range(max(nphase, 1))]
         _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
-        plan = M.Plan(bufs)
-        assert plan.stats()["positional"] == (values == "stencil")
-        for graph in (False, True):
-            vv = zeros_dev(n * (k + 1))
-            plan.exec(to_dev(x), vv, use_graph=graph)
-            assert np.array_equal(to_host(vv), gold), (nphase, graph)
+        for flags in (0, M.Plan.WIDE):
+            plan = M.Plan(bufs, flags)
+            assert plan.stats()["positional"] == (values == "stencil")
+            assert (plan.stats()["compact_launches"] > 0) == (flags == 0)
+            for graph in (False, True):
+                vv = zeros_dev(n * (k + 1))
+                plan.exec(to_dev(x), vv, use_graph=graph)
+                assert np.array_equal(to_host(vv), gold), (nphase, flags, graph)
 
 
 def _rel_errs_scaled(out32, gold, n, k, sigma):
