@@ -9,9 +9,13 @@
 //      (the plan computes s powers of r; the last one is not used),
 //   2. forms the Gram matrix G = Y^T Y of Y = [P, R] (m = 2s+1 columns) in
 //      one pass over HBM,
-//   3. takes s CG steps on length-m coordinate vectors on the host (the
-//      shift B maps column j of a block to column j+1),
+//   3. takes s CG steps on length-m coordinate vectors in one small
+//      workgroup (the shift B maps column j of a block to column j+1),
 //   4. recovers x += Y x', r = Y r', p = Y p' in one pass.
+// Nothing waits for the host inside an iteration: the coefficients stay on
+// the GPU, and the host reads the convergence flag of iteration i while
+// iteration i + 1 runs.  An iteration launched after convergence changes
+// nothing (its recovery pass is skipped).
 // The operator is A/sigma (sigma = the power of two at or above the largest
 // row abs-sum, an exact scaling) applied to b/sigma, which has the same
 // solution and keeps the monomial basis from growing.
@@ -28,11 +32,22 @@ namespace {
 
 constexpr int SS_MAX_S = 8;
 constexpr int SS_MAXM = 2 * SS_MAX_S + 1;
-constexpr int GT = 256, GR = 64;  // threads, rows per Gram tile
+constexpr int GT = 256, GR = 256;  // threads, rows per Gram tile
 constexpr int GRID = 1024;
 
 struct Coef {
   double c[3][SS_MAXM];  // x, r, p
+};
+
+// Solver state in device memory, written by k_start and k_sstep_coef.
+struct SsState {
+  double rr0;  // squared residual norm at the first iteration (< 0 before)
+  double rel;  // relative residual estimate after the last active iteration
+  double bad;  // p'Ap when the recurrence broke down
+  int done;    // converged or broke down: later iterations change nothing
+  int fail;    // p'Ap <= 0
+  int outer;   // outer iterations whose recovery ran
+  int active;  // this iteration's recovery runs
 };
 
 __device__ __forceinline__ double yval(const double *vp, int np, const double *vr, long n, int v, long row) {
@@ -83,19 +98,112 @@ __global__ __launch_bounds__(GT) void k_gram_finish(const double *__restrict__ p
   if (t == 0) g[q] = red[0];
 }
 
-// x += Y c[0], r = Y c[1], p = Y c[2]
+// a^T G c for length-m coordinate vectors in LDS: lane i forms
+// u_i = sum_j G[i][j] c[j] in order j = 0..m-1, then every lane sums
+// a[i] u_i in order i = 0..m-1 (a uniform result).
+__device__ double gdot(const double (*G)[SS_MAXM], const double *a, const double *c, double *u, int m) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  if (t < m) {
+    double acc = 0.0;
+    for (int j = 0; j < m; j++) acc += G[t][j] * c[j];
+    u[t] = acc;
+  }
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < m; i++) s += a[i] * u[i];
+  return s;
+}
+
+// The s CG steps of one outer iteration on the coordinates (one workgroup
+// of 64 lanes; lane i owns element i of every coordinate vector).
+// Coordinates: p' = e_0, r' = e_(s+1), x' = 0; B shifts within each block.
+__global__ __launch_bounds__(64) void k_sstep_coef(const double *__restrict__ gp, int s, double rtol,
+                                                   SsState *st, Coef *coef) {
+  __shared__ double G[SS_MAXM][SS_MAXM];
+  __shared__ double xc[SS_MAXM], rc[SS_MAXM], pc[SS_MAXM], bp[SS_MAXM], rn[SS_MAXM], u[SS_MAXM];
+  const int t = threadIdx.x, m = 2 * s + 1;
+  if (st->done) {
+    if (t == 0) st->active = 0;
+    return;
+  }
+  for (int q = t; q < m * (m + 1) / 2; q += 64) {  // packed upper triangle, row-major
+    int i = 0, k = q;
+    while (k >= m - i) k -= m - i, i++;
+    G[i][i + k] = G[i + k][i] = gp[q];
+  }
+  if (t < m) {
+    xc[t] = 0.0;
+    rc[t] = t == s + 1 ? 1.0 : 0.0;
+    pc[t] = t == 0 ? 1.0 : 0.0;
+  }
+  double rr = gdot(G, rc, rc, u, m);
+  double rr0 = st->rr0;
+  __syncthreads();  // every lane has read rr0
+  if (rr0 < 0) rr0 = rr;
+  if (t == 0) st->rr0 = rr0;
+  if (rr <= 0.0 || sqrt(rr / rr0) <= rtol) {
+    if (t == 0) {
+      st->rel = rr0 > 0 ? sqrt(fmax(rr, 0.0) / rr0) : 0.0;
+      st->done = 1;
+      st->active = 0;
+    }
+    return;
+  }
+  for (int j = 0; j < s; j++) {
+    __syncthreads();  // pc of the previous step, written by every lane
+    if (t < m) bp[t] = (t >= 1 && t <= s) || (t >= s + 2 && t <= 2 * s) ? pc[t - 1] : 0.0;
+    const double pap = gdot(G, pc, bp, u, m);
+    if (!(pap > 0.0)) {
+      if (t == 0) {
+        st->bad = pap;
+        st->fail = 1;
+        st->done = 1;
+        st->active = 0;
+      }
+      return;
+    }
+    const double alpha = rr / pap;
+    if (t < m) {
+      xc[t] += alpha * pc[t];
+      rn[t] = rc[t] - alpha * bp[t];
+    }
+    const double rr1 = gdot(G, rn, rn, u, m);
+    const double beta = rr1 / rr;
+    if (t < m) {
+      pc[t] = rn[t] + beta * pc[t];
+      rc[t] = rn[t];
+    }
+    rr = rr1;
+  }
+  if (t < m) {
+    coef->c[0][t] = xc[t];
+    coef->c[1][t] = rc[t];
+    coef->c[2][t] = pc[t];
+  }
+  if (t == 0) {
+    const double rel = sqrt(fmax(rr, 0.0) / rr0);
+    st->rel = rel;
+    st->outer += 1;
+    st->active = 1;
+    st->done = rel <= rtol ? 1 : 0;
+  }
+}
+
+// x += Y c[0], r = Y c[1], p = Y c[2], when this iteration is active
 __global__ __launch_bounds__(256) void k_combine3(const double *__restrict__ vp, int np, const double *__restrict__ vr,
-                                                  int nr, long n, Coef c, double *__restrict__ x,
+                                                  int nr, long n, const Coef *__restrict__ c,
+                                                  const SsState *__restrict__ st, double *__restrict__ x,
                                                   double *__restrict__ r, double *__restrict__ p) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || !st->active) return;
   const int m = np + nr;
   double sx = 0.0, sr = 0.0, sp = 0.0;
   for (int v = 0; v < m; v++) {
     const double y = yval(vp, np, vr, n, v, i);
-    sx = fma(c.c[0][v], y, sx);
-    sr = fma(c.c[1][v], y, sr);
-    sp = fma(c.c[2][v], y, sp);
+    sx = fma(c->c[0][v], y, sx);
+    sr = fma(c->c[1][v], y, sr);
+    sp = fma(c->c[2][v], y, sp);
   }
   x[i] += sx;
   r[i] = sr;
@@ -103,8 +211,9 @@ __global__ __launch_bounds__(256) void k_combine3(const double *__restrict__ vp,
 }
 
 __global__ void k_start(const double *__restrict__ b, double inv_sigma, long n, double *__restrict__ x,
-                        double *__restrict__ r, double *__restrict__ p) {
+                        double *__restrict__ r, double *__restrict__ p, SsState *st) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *st = SsState{-1.0, 1.0, 0.0, 0, 0, 0, 0};
   if (i >= n) return;
   const double v = b[i] * inv_sigma;
   x[i] = 0.0;
@@ -119,8 +228,15 @@ struct mpkx_sstep {
   double sigma = 1.0;
   mpkx_fused *mpk = nullptr;
   DevArray<double> vp, vr, r, p, partial, gram;
+  DevArray<SsState> state;
+  DevArray<Coef> coef;
+  int *done_host = nullptr;  // pinned: the done flag of the last two iterations
+  hipEvent_t ev[2] = {nullptr, nullptr};
   ~mpkx_sstep() {
     if (mpk) mpkx_fused_destroy(mpk);
+    if (done_host) (void)hipHostFree(done_host);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
   }
 };
 
@@ -157,8 +273,10 @@ int mpkx_sstep_create(mpkx_sstep **out, int n, const int *ptr, const int *col, c
   const int m = 2 * s + 1, npairs = m * (m + 1) / 2;
   if ((rc = S->vp.alloc((size_t)(s + 1) * n)) || (rc = S->vr.alloc((size_t)(s + 1) * n)) ||
       (rc = S->r.alloc(n)) || (rc = S->p.alloc(n)) || (rc = S->partial.alloc((size_t)GRID * npairs)) ||
-      (rc = S->gram.alloc(npairs)))
+      (rc = S->gram.alloc(npairs)) || (rc = S->state.alloc(1)) || (rc = S->coef.alloc(1)))
     return rc;
+  MPKX_HIP(hipHostMalloc((void **)&S->done_host, 2 * sizeof(int), hipHostMallocDefault));
+  for (hipEvent_t &e : S->ev) MPKX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   *out = guard.release();
   return MPKX_OK;
 }
@@ -173,12 +291,9 @@ int mpkx_sstep_solve(mpkx_sstep *S, const double *b, double *x, double rtol, int
   const int s = S->s, m = 2 * s + 1, npairs = m * (m + 1) / 2, np = s + 1, nr = s;
   const long n = S->n;
   const unsigned nb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_start, dim3(nb), dim3(256), 0, st, b, 1.0 / S->sigma, n, x, S->r.p, S->p.p);
+  hipLaunchKernelGGL(k_start, dim3(nb), dim3(256), 0, st, b, 1.0 / S->sigma, n, x, S->r.p, S->p.p, S->state.p);
   MPKX_HIP(hipGetLastError());
-  std::vector<double> gp(npairs), G((size_t)m * m);
-  double rr0 = -1.0, rel = 1.0;
-  int outer = 0;
-  for (; outer < max_outer; outer++) {
+  for (int it = 0; it < max_outer; it++) {
     int rc;
     // eager launches: the two calls alternate their (x, vv) pointers, which
     // would re-record a cached graph every time
@@ -186,67 +301,24 @@ int mpkx_sstep_solve(mpkx_sstep *S, const double *b, double *x, double rtol, int
       return rc;
     hipLaunchKernelGGL(k_gram_partial, dim3(GRID), dim3(GT), 0, st, S->vp.p, np, S->vr.p, nr, n, S->partial.p);
     hipLaunchKernelGGL(k_gram_finish, dim3(npairs), dim3(GT), 0, st, S->partial.p, GRID, npairs, S->gram.p);
+    hipLaunchKernelGGL(k_sstep_coef, dim3(1), dim3(64), 0, st, S->gram.p, s, rtol, S->state.p, S->coef.p);
+    hipLaunchKernelGGL(k_combine3, dim3(nb), dim3(256), 0, st, S->vp.p, np, S->vr.p, nr, n, S->coef.p, S->state.p,
+                       x, S->r.p, S->p.p);
     MPKX_HIP(hipGetLastError());
-    MPKX_HIP(hipMemcpyAsync(gp.data(), S->gram.p, sizeof(double) * npairs, hipMemcpyDeviceToHost, st));
-    MPKX_HIP(hipStreamSynchronize(st));
-    for (int i = 0, q = 0; i < m; i++)
-      for (int j = i; j < m; j++, q++) G[(size_t)i * m + j] = G[(size_t)j * m + i] = gp[q];
-    // coordinates: p' = e_0, r' = e_{s+1}, x' = 0; B shifts within each block
-    std::vector<double> xc(m, 0.0), rc_(m, 0.0), pc(m, 0.0), bp(m), rn(m);
-    pc[0] = 1.0;
-    rc_[s + 1] = 1.0;
-    auto shift = [&](const std::vector<double> &v, std::vector<double> &out) {
-      std::fill(out.begin(), out.end(), 0.0);
-      for (int j = 0; j < s; j++) out[j + 1] = v[j];           // P block: columns 0..s
-      for (int j = s + 1; j < 2 * s; j++) out[j + 1] = v[j];   // R block: columns s+1..2s
-    };
-    auto gdot = [&](const std::vector<double> &a, const std::vector<double> &c) {
-      double t = 0.0;
-      for (int i = 0; i < m; i++) {
-        double u = 0.0;
-        for (int j = 0; j < m; j++) u += G[(size_t)i * m + j] * c[j];
-        t += a[i] * u;
-      }
-      return t;
-    };
-    double rr = gdot(rc_, rc_);
-    if (rr0 < 0) rr0 = rr;
-    if (rr <= 0.0 || std::sqrt(rr / rr0) <= rtol) {
-      rel = rr0 > 0 ? std::sqrt(std::max(rr, 0.0) / rr0) : 0.0;
-      break;
-    }
-    for (int j = 0; j < s; j++) {
-      shift(pc, bp);
-      const double pap = gdot(pc, bp);
-      if (!(pap > 0.0)) return fail(MPKX_EINVAL, "sstep_solve: p'Ap = %g (matrix not SPD or basis lost)", pap);
-      const double alpha = rr / pap;
-      for (int i = 0; i < m; i++) {
-        xc[i] += alpha * pc[i];
-        rn[i] = rc_[i] - alpha * bp[i];
-      }
-      const double rr1 = gdot(rn, rn);
-      const double beta = rr1 / rr;
-      for (int i = 0; i < m; i++) pc[i] = rn[i] + beta * pc[i];
-      rc_ = rn;
-      rr = rr1;
-    }
-    Coef c{};
-    for (int i = 0; i < m; i++) {
-      c.c[0][i] = xc[i];
-      c.c[1][i] = rc_[i];
-      c.c[2][i] = pc[i];
-    }
-    hipLaunchKernelGGL(k_combine3, dim3(nb), dim3(256), 0, st, S->vp.p, np, S->vr.p, nr, n, c, x, S->r.p, S->p.p);
-    MPKX_HIP(hipGetLastError());
-    rel = std::sqrt(std::max(rr, 0.0) / rr0);
-    if (rel <= rtol) {
-      outer++;
-      break;
+    MPKX_HIP(hipMemcpyAsync(&S->done_host[it & 1], &S->state.p->done, sizeof(int), hipMemcpyDeviceToHost, st));
+    MPKX_HIP(hipEventRecord(S->ev[it & 1], st));
+    // the previous iteration's flag, read while this one runs
+    if (it >= 1) {
+      MPKX_HIP(hipEventSynchronize(S->ev[(it - 1) & 1]));
+      if (S->done_host[(it - 1) & 1]) break;
     }
   }
+  SsState fin;
+  MPKX_HIP(hipMemcpyAsync(&fin, S->state.p, sizeof(SsState), hipMemcpyDeviceToHost, st));
   MPKX_HIP(hipStreamSynchronize(st));
-  if (iters) *iters = outer * s;
-  if (resid) *resid = rel;
+  if (fin.fail) return fail(MPKX_EINVAL, "sstep_solve: p'Ap = %g (matrix not SPD or basis lost)", fin.bad);
+  if (iters) *iters = fin.outer * s;
+  if (resid) *resid = fin.rel;
   return MPKX_OK;
 }
 
