@@ -40,6 +40,28 @@ def test_sstep_cg_matches_classical_cg(s, lpl):
     assert it_cg - s <= it <= 1.25 * it_cg + s
 
 
+def test_sstep_repeated_solves_match():
+    """Solving again into the same x (the state, flags and coefficients
+    left by the previous solve are reset) gives the same bits, iteration
+    count and residual estimate."""
+    say("s-step CG, repeated solves")
+    rp, ci, va, b = _problem(24)
+    n = len(rp) - 1
+    part = M.partition_grid([24, 24, 24], [2, 2, 2])
+    S = M.SStepCG(rp, ci, va, part, 4, 2)
+    x = zeros_dev(n)
+    bd = to_dev(b)
+    runs = []
+    for _ in range(3):
+        it, res = S.solve(bd, x, rtol=1e-9)
+        runs.append((it, res, to_host(x).copy()))
+    for it, res, xh in runs[1:]:
+        assert it == runs[0][0] and res == runs[0][1]
+        assert np.array_equal(xh, runs[0][2])
+    x_cg, _ = O.cg(rp, ci, va, b, 1e-9)
+    assert np.linalg.norm(runs[0][2] - x_cg) / np.linalg.norm(x_cg) <= 1e-6
+
+
 def test_sstep_cg_errors():
     rp, ci, va, b = _problem(8)
     part = np.zeros(len(rp) - 1, np.int32)
