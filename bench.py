@@ -402,7 +402,18 @@ def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
         st = plan.stats()
         rec = {"npart": P, "nphase": nphase, "ms_per_step": round(ms, 5), "vs_kxspmv": round(ms_plain / ms, 3),
                "redundancy": round(st["rows"] / (len(x) * k), 3), "ghosts": st["ghosts"],
-               "prep_seconds": round(t_prep, 2)}
+               "compact_launches": st["compact_launches"], "prep_seconds": round(t_prep, 2)}
+        if P == 64 and nphase == 0:  # the same plan collecting in a separate pass (collect_results)
+            del plan
+            plan = M.Plan(bufs, M.Plan.GATHER)
+            plan.exec(xd, vv)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(steps):
+                plan.exec(xd, vv)
+            e1.record()
+            torch.cuda.synchronize()
+            rec["gather_collect_ms"] = round(e0.elapsed_time(e1) / steps, 5)
         del plan, bufs
         if P == 64 and nphase <= 1:  # the shared-memory task lists of the same schedule (mpk2/mpkprep.c)
             tcount, tidx = M.prep_mpk(rp, ci, k, nphase, parts, levels, sk)

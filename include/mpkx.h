@@ -224,9 +224,11 @@ int mpkx_plan_create(mpkx_plan **plan, mpkx_buffers *const *bufs, int npart);
  * MPKX_PLAN_WIDE keeps 64-bit row pointers, 32-bit columns and one output
  * slot per row for every launch (otherwise used only by the launches the
  * compact form does not fit: 32-bit row pointers, 16-bit column offsets
- * per workgroup, one output base per workgroup).  Results are the same
- * bits either way. */
-enum { MPKX_PLAN_EXPLICIT = 1, MPKX_PLAN_WIDE = 2 };
+ * per workgroup, one output base per workgroup).  mpkx_exec writes each
+ * value of levels >= 1 into vv from the launch that computes it (one
+ * designated copy per value); MPKX_PLAN_GATHER collects them afterwards
+ * instead, like collect_results.  Results are the same bits either way. */
+enum { MPKX_PLAN_EXPLICIT = 1, MPKX_PLAN_WIDE = 2, MPKX_PLAN_GATHER = 4 };
 int mpkx_plan_create_opts(mpkx_plan **plan, mpkx_buffers *const *bufs,
                           int npart, int flags);
 /* positional: 1 when the plan keeps positional values; matrix_bytes:

@@ -535,6 +535,7 @@ class Plan:
 
     EXPLICIT = 1  # keep one value per entry (no positional values)
     WIDE = 2  # 64-bit row pointers, 32-bit columns, an output slot per row (no compact structure)
+    GATHER = 4  # exec collects the Krylov block in a separate pass (collect_results) instead of directly
 
     def __init__(self, bufset, flags=0):
         self._h = C.c_void_p()

@@ -39,11 +39,13 @@ constexpr int POS_MAX = 64;   // positional values: longest row
 constexpr int CS_BATCH = 8;   // entries per thread in flight while staging
 
 // POS: va holds the POS_MAX-or-fewer positional values, not one per entry.
+// vout (optional): the Krylov block being collected; row r also writes
+// vout[dst2[r]] when dst2[r] >= 0 (the row is its value's designated copy).
 template <typename PtrT, bool INDIRECT, bool POS = false>
 __global__ __launch_bounds__(CS_THREADS) void k_csr_stream(
     const PtrT *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ va,
     const double *x, double *y, const int *__restrict__ out_row, const int *__restrict__ blk,
-    int blk0) {
+    int blk0, const int *__restrict__ dst2 = nullptr, double *__restrict__ vout = nullptr) {
   __shared__ double prod[CS_CAP];
   __shared__ double pv[POS ? POS_MAX : 1];
   const int b = blk0 + blockIdx.x;
@@ -87,11 +89,13 @@ __global__ __launch_bounds__(CS_THREADS) void k_csr_stream(
       for (PtrT j = jr; j < e; j++)
         s = __dadd_rn(s, POS ? __dmul_rn(pv[j - jr], prod[j - j0]) : prod[j - j0]);
       y[INDIRECT ? out_row[r] : r] = s;
+      if (vout && dst2[r] >= 0) vout[dst2[r]] = s;
     }
   } else if (threadIdx.x == 0) {  // a single row longer than the LDS stage
     double s = 0.0;
     for (PtrT j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(POS ? va[j - j0] : va[j], x[ci[j]]));
     y[INDIRECT ? out_row[r0] : r0] = s;
+    if (vout && dst2[r0] >= 0) vout[dst2[r0]] = s;
   }
 }
 
@@ -106,7 +110,7 @@ template <bool POS>
 __global__ __launch_bounds__(CS_THREADS) void k_csr_compact(
     const int *__restrict__ rp, const uint16_t *__restrict__ ci, const double *__restrict__ va,
     const double *x, double *y, const int *__restrict__ banchor, const int *__restrict__ bout,
-    const int *__restrict__ blk, int blk0) {
+    const int *__restrict__ blk, int blk0, const int *__restrict__ dst2, double *__restrict__ vout) {
   __shared__ double prod[CS_CAP];
   __shared__ double pv[POS ? POS_MAX : 1];
   const int b = blk0 + blockIdx.x;
@@ -149,11 +153,13 @@ __global__ __launch_bounds__(CS_THREADS) void k_csr_compact(
       const int jr = rp[r], e = rp[r + 1];
       for (int j = jr; j < e; j++) s = __dadd_rn(s, POS ? __dmul_rn(pv[j - jr], prod[j - j0]) : prod[j - j0]);
       y[o0 + (r - r0)] = s;
+      if (vout && dst2[r] >= 0) vout[dst2[r]] = s;
     }
   } else if (threadIdx.x == 0) {  // a single row longer than the LDS stage
     double s = 0.0;
     for (int j = j0; j < j1; j++) s = __dadd_rn(s, __dmul_rn(POS ? va[j - j0] : va[j], xa[ci[j]]));
     y[o0] = s;
+    if (vout && dst2[r0] >= 0) vout[dst2[r0]] = s;
   }
 }
 
@@ -228,6 +234,10 @@ struct mpkx_plan {
   // compact structure (k_csr_compact) of the groups that fit it; rp, ci and
   // out_row are kept only when some group does not
   int compact_groups = 0;
+  // direct collection: dst2[row] = the Krylov-block index the row writes
+  // (its value's designated copy) or -1; level 0 is x itself
+  bool direct = false;
+  DevArray<int> dst2;
   DevArray<int> rp32, banchor, bout;
   DevArray<uint16_t> ci16;
   long rows = 0, nnz = 0, ghosts = 0;
@@ -270,6 +280,7 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
   // Rows grouped by (phase, level); inside a group by partition then row.
   std::vector<long> rp(1, 0);
   std::vector<int> ci, outr, blk;
+  std::vector<long> rowv;  // the Krylov-block index l*n + i of each row
   std::vector<double> va;
   int gblk = 0;  // workgroups of the groups so far
   try {
@@ -308,6 +319,7 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
             }
             rp.push_back((long)ci.size());
             outr.push_back((int)(pl->base[p] + b->mbuf_offsets[ph] + m));
+            rowv.push_back(im[m]);
           }
           // workgroups never span two partitions (consecutive output slots)
           make_blocks(rp, rp0, (int)rp.size() - 1, blk);
@@ -408,6 +420,25 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
   for (size_t v = 0; v < csrc.size(); v++)
     if (csrc[v] < 0)
       return fail(MPKX_ESTRUCT, "plan_create: level %zu row %zu is computed by no partition", v / n, v % n);
+  // Direct collection: the first row computing each value of levels >= 1
+  // writes it into the Krylov block as well (every copy of a value is
+  // bitwise equal), replacing the gather over csrc.
+  std::vector<int> dst2;
+  {
+    std::vector<char> have(csrc.size(), 0);
+    dst2.assign(rowv.size(), -1);
+    long covered = 0;
+    for (size_t r = 0; r < rowv.size(); r++) {
+      const long v = rowv[r];
+      if (v >= n && v < (long)csrc.size() && !have[v]) {
+        have[v] = 1;
+        dst2[r] = (int)v;
+        covered++;
+      }
+    }
+    pl->direct = !(flags & MPKX_PLAN_GATHER) && covered == (long)csrc.size() - n;
+    if (!pl->direct) dst2.clear();
+  }
 
   // Compact structure for a (phase, level) group when every workgroup's
   // outputs are consecutive and its inputs lie within 65536 slots of its
@@ -458,7 +489,7 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
   if ((rc = pl->rp.upload(rp)) || (rc = pl->ci.upload(ci)) || (rc = pl->va.upload(va)) ||
       (rc = pl->out_row.upload(outr)) || (rc = pl->blk.upload(blk)) || (rc = pl->gdst.upload(gdst)) ||
       (rc = pl->gsrc.upload(gsrc)) || (rc = pl->xdst.upload(xdst)) || (rc = pl->xsrc.upload(xsrc)) ||
-      (rc = pl->csrc.upload(csrc)) || (rc = pl->vv.alloc(pl->flat)) ||
+      (rc = pl->csrc.upload(csrc)) || (rc = pl->dst2.upload(dst2)) || (rc = pl->vv.alloc(pl->flat)) ||
       (rc = pl->stage.alloc(std::max<long>(pl->ghosts, 1))))
     return rc;
   *out = guard.release();
@@ -506,7 +537,9 @@ int mpkx_plan_stats(const mpkx_plan *pl, long *rows, long *nnz, long *ghosts, in
 }  // extern "C"
 
 // Phases: do_task(0); then (ghost gather, do_task) per phase.
-static int run_phases(mpkx_plan *pl, hipStream_t st) {
+// vout: collect every value of levels >= 1 into this Krylov block on the
+// way (direct plans), or null.
+static int run_phases(mpkx_plan *pl, hipStream_t st, double *vout = nullptr) {
   size_t gi = 0;
   for (int ph = 0; ph <= pl->nphase; ph++) {
     if (ph > 0) {
@@ -525,18 +558,19 @@ static int run_phases(mpkx_plan *pl, hipStream_t st) {
     }
     for (; gi < pl->groups.size() && pl->groups[gi].ph == ph; gi++) {
       const auto &g = pl->groups[gi];
+      const int *d2 = vout ? pl->dst2.p : nullptr;
       if (g.compact && pl->positional)
         hipLaunchKernelGGL((k_csr_compact<true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp32.p, pl->ci16.p,
-                           pl->va.p, pl->vv.p, pl->vv.p, pl->banchor.p, pl->bout.p, pl->blk.p, g.blk0);
+                           pl->va.p, pl->vv.p, pl->vv.p, pl->banchor.p, pl->bout.p, pl->blk.p, g.blk0, d2, vout);
       else if (g.compact)
         hipLaunchKernelGGL((k_csr_compact<false>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp32.p, pl->ci16.p,
-                           pl->va.p, pl->vv.p, pl->vv.p, pl->banchor.p, pl->bout.p, pl->blk.p, g.blk0);
+                           pl->va.p, pl->vv.p, pl->vv.p, pl->banchor.p, pl->bout.p, pl->blk.p, g.blk0, d2, vout);
       else if (pl->positional)
         hipLaunchKernelGGL((k_csr_stream<long, true, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
-                           pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
+                           pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0, d2, vout);
       else
         hipLaunchKernelGGL((k_csr_stream<long, true>), dim3(g.nblk), dim3(CS_THREADS), 0, st, pl->rp.p,
-                           pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0);
+                           pl->ci.p, pl->va.p, pl->vv.p, pl->vv.p, pl->out_row.p, pl->blk.p, g.blk0, d2, vout);
     }
   }
   MPKX_HIP(hipGetLastError());
@@ -572,6 +606,10 @@ static int exec_launches(mpkx_plan *pl, const double *x, double *vv, hipStream_t
   if (nx > 0)
     hipLaunchKernelGGL(k_load_pairs, dim3(grid1(nx, 256)), dim3(256), 0, st, pl->vv.p, x,
                        pl->xdst.p, pl->xsrc.p, nx);
+  if (pl->direct) {  // level 0 is x; the level launches write the rest
+    hipLaunchKernelGGL(k_dcopy, dim3(grid1(pl->n, 256)), dim3(256), 0, st, vv, x, (long)pl->n);
+    return run_phases(pl, st, vv);
+  }
   int rc = run_phases(pl, st);
   if (rc) return rc;
   long nv = (long)pl->csrc.count;

@@ -42,7 +42,8 @@ def test_exact_plan_bitwise_reference(case):
     # entry both give the reference's bits
     # and so do the compact structure (16-bit column offsets) and the wide one
     for flags, positional in ((M.Plan.WIDE, _positional(rp, va)), (M.Plan.EXPLICIT | M.Plan.WIDE, False),
-                              (M.Plan.EXPLICIT, False), (0, _positional(rp, va))):
+                              (M.Plan.EXPLICIT, False), (M.Plan.GATHER, _positional(rp, va)),
+                              (0, _positional(rp, va))):
         plan = M.Plan(bufs, flags)
         assert plan.stats()["positional"] == positional
         assert (plan.stats()["compact_launches"] > 0) == (not flags & M.Plan.WIDE)
@@ -389,11 +390,11 @@ def test_exact_plan_headline_size_bitwise_oracle():
     gold = O.spmv_k(rp, ci, va, x, k)
     parts = [M.partition_grid([N] * 3, [4, 4, 4], [16 * ph] * 3) for ph in range(nphase)]
     _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
-    for flags in (0, M.Plan.WIDE):
+    for flags in (0, M.Plan.WIDE, M.Plan.GATHER):
         plan = M.Plan(bufs, flags)
         # phase 0 fits the compact structure; later phases read both earlier
         # rows and received ghosts, which may lie too far apart
-        assert (plan.stats()["compact_launches"] > 0) == (flags == 0)
+        assert (plan.stats()["compact_launches"] > 0) == (flags != M.Plan.WIDE)
         vv = zeros_dev(n * (k + 1))
         plan.exec(to_dev(x), vv)
         assert np.array_equal(to_host(vv), gold), flags
@@ -434,10 +435,10 @@ def test_exact_plan_cube64_against_oracle(values):
     for nphase in (0, 1):
         parts = [M.partition_grid([64] * 3, [2, 2, 2], [16 * ph] * 3) for ph in range(max(nphase, 1))]
         _, _, bufs, _ = M.prepare(rp, ci, va, k, nphase, parts)
-        for flags in (0, M.Plan.WIDE):
+        for flags in (0, M.Plan.WIDE, M.Plan.GATHER):
             plan = M.Plan(bufs, flags)
             assert plan.stats()["positional"] == (values == "stencil")
-            assert (plan.stats()["compact_launches"] > 0) == (flags == 0)
+            assert (plan.stats()["compact_launches"] > 0) == (flags != M.Plan.WIDE)
             for graph in (False, True):
                 vv = zeros_dev(n * (k + 1))
                 plan.exec(to_dev(x), vv, use_graph=graph)
