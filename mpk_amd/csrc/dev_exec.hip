@@ -169,7 +169,15 @@ __global__ void k_copy_pairs(double *flat, const int *__restrict__ dst, const in
   if (t < count) flat[dst[t]] = flat[src[t]];
 }
 
+// The HBM counter calibration copy (mpkx_dcopy): no other caller, so its
+// per-dispatch counter readings always stand for a copy of known size.
 __global__ void k_dcopy(double *__restrict__ dst, const double *__restrict__ src, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+// Level 0 of the Krylov block (x) in a direct-collecting exec.
+__global__ void k_copy_level0(double *__restrict__ dst, const double *__restrict__ src, long n) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = src[t];
 }
@@ -607,7 +615,7 @@ static int exec_launches(mpkx_plan *pl, const double *x, double *vv, hipStream_t
     hipLaunchKernelGGL(k_load_pairs, dim3(grid1(nx, 256)), dim3(256), 0, st, pl->vv.p, x,
                        pl->xdst.p, pl->xsrc.p, nx);
   if (pl->direct) {  // level 0 is x; the level launches write the rest
-    hipLaunchKernelGGL(k_dcopy, dim3(grid1(pl->n, 256)), dim3(256), 0, st, vv, x, (long)pl->n);
+    hipLaunchKernelGGL(k_copy_level0, dim3(grid1(pl->n, 256)), dim3(256), 0, st, vv, x, (long)pl->n);
     return run_phases(pl, st, vv);
   }
   int rc = run_phases(pl, st);
