@@ -544,10 +544,11 @@ def test_task_lists_exec_bitwise_reference(case):
     assert np.allclose(to_host(ones), 1.0, rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("ci_case", [0, 3, 4, 5])
+@pytest.mark.parametrize("ci_case", [0, 2, 3, 4, 5])
 def test_pdmpk_gpu_bitwise_oracle(ci_case, tmp_path):
     """pdmpk buffers executed on the GPU equal the oracle exec bitwise (same
-    partial-sum order) and k x SpMV to 1e-13."""
+    partial-sum order) and k x SpMV to 1e-13, for stencil values and the
+    random 9-point values (case 2)."""
     from tests.test_pdmpk import CASES, pd_case
     case = CASES[ci_case]
     say("pdmpk", case)
