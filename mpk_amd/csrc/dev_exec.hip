@@ -447,6 +447,7 @@ int mpkx_plan_create_opts(mpkx_plan **out, mpkx_buffers *const *bufs, int npart,
     pl->direct = !(flags & MPKX_PLAN_GATHER) && covered == (long)csrc.size() - n;
     if (!pl->direct) dst2.clear();
   }
+  if (pl->direct) std::vector<int>().swap(csrc);  // the collection gather is not needed
 
   // Compact structure for a (phase, level) group when every workgroup's
   // outputs are consecutive and its inputs lie within 65536 slots of its
