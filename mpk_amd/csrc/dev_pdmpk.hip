@@ -209,7 +209,7 @@ int mpkx_pd_plan_create(mpkx_pd_plan **out, const mpkx_pd *pd) {
         long start = rp[r];
         int e = r + 1;
         if (rp[e] - start <= PCAP)
-          while (e < r1 && rp[e + 1] - start <= PCAP && e - r < 4 * PT) e++;
+          while (e < r1 && rp[e + 1] - start <= PCAP && e - r < PT) e++;
         r = e;
       }
       pl->nblk.push_back((int)blk.size() - pl->blk0.back());
