@@ -782,7 +782,14 @@ int mpkx_tasks_create(mpkx_tasks **out, int n, const int *ptr, const int *col, c
   std::vector<long> rp(1, 0);
   std::vector<int> ci, outr, blk;
   std::vector<double> va;
+  // The lists hold the skirt tasks of every partition, so a value is often
+  // computed by several tasks, all from the same inputs and all writing the
+  // same slot of the one Krylov block.  Only the first task (in launch
+  // order) of each value is launched: it runs no later than any other copy,
+  // so every reader finds the value, with the same bits.
+  std::vector<char> have;
   try {
+    have.assign((size_t)n * (nlevel + 1), 0);
     for (int ph = 0; ph <= nphase; ph++) {
       std::vector<long> cur(npart);
       for (int p = 0; p < npart; p++) cur[p] = start[(size_t)ph * npart + p];
@@ -792,6 +799,10 @@ int mpkx_tasks_create(mpkx_tasks **out, int n, const int *ptr, const int *col, c
           long end = start[(size_t)ph * npart + p + 1];
           long &c = cur[p];
           for (; c < end && tidx[c] / n == l; c++) {
+            if (tidx[c] < 0 || tidx[c] >= (long)have.size())
+              return fail(MPKX_EINVAL, "tasks_create: task %ld out of range", tidx[c]);
+            if (have[tidx[c]]) continue;
+            have[tidx[c]] = 1;
             long i = tidx[c] % n;
             double inv = 1.0 / (ptr[i + 1] - ptr[i]);  // exec_mpk's 1/deg (mpk2/mpkexec.c:28)
             for (int j = ptr[i]; j < ptr[i + 1]; j++) {
