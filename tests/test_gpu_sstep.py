@@ -62,6 +62,16 @@ def test_sstep_repeated_solves_match():
     assert np.linalg.norm(runs[0][2] - x_cg) / np.linalg.norm(x_cg) <= 1e-6
 
 
+def test_sstep_not_spd_reports_breakdown():
+    """A negative definite matrix gives p'Ap < 0 in the first step, found
+    on the GPU: the solve returns an error instead of a result."""
+    rp, ci, va, b = _problem(8)
+    part = M.partition_grid([8, 8, 8], [2, 2, 2])
+    S = M.SStepCG(rp, ci, -va, part, 2)
+    with pytest.raises(M.MpkxError, match="SPD"):
+        S.solve(to_dev(b), zeros_dev(len(b)), rtol=1e-8)
+
+
 def test_sstep_cg_errors():
     rp, ci, va, b = _problem(8)
     part = np.zeros(len(rp) - 1, np.int32)
