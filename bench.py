@@ -372,6 +372,27 @@ def precision_form(M, torch, dev, N=2048, k=16, blocks=(64, 64), s=3, blocks32=(
     return out
 
 
+def exec_bufs_host(plan, bufs, x, P, k, B, reps=3):
+    """The host drop-in mpkx_exec_bufs: every rank's vv_buf[0 .. rcount[0])
+    loaded with x[idx_buf] as the reference main does, copied to the GPU,
+    every phase run, every vv_buf copied back (PCIe included)."""
+    bufs.alloc_val_bufs()
+    for p in range(P):
+        b = bufs[p]
+        idx = np.ctypeslib.as_array(b.idx_buf, (b.buf_count,))
+        vals = np.ctypeslib.as_array(b.vv_buf, (b.buf_count,))
+        vals[:b.rcount[0]] = x[idx[:b.rcount[0]]]
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan.exec_bufs(bufs)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    moved = sum(8 * (bufs[p].rcount[0] + bufs[p].buf_count) for p in range(P))
+    return {"ms_per_step": round(best * 1e3, 3), "value": round(k * B / best / 1e9, 2), "unit": "GB/s",
+            "pcie_bytes": int(moved), "note": "host vv_bufs in and out, min of %d" % reps}
+
+
 def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
                    cases=((64, 0), (64, 1), (64, 2), (64, 3), (512, 0), (512, 3))):
     """The reference's own mpk2 schedule on the GPU (mpkx_plan: the phases'
@@ -403,6 +424,11 @@ def exact_schedule(M, torch, dev, rp, ci, va, x, N, k, ms_plain, steps,
         rec = {"npart": P, "nphase": nphase, "ms_per_step": round(ms, 5), "vs_kxspmv": round(ms_plain / ms, 3),
                "redundancy": round(st["rows"] / (len(x) * k), 3), "ghosts": st["ghosts"],
                "compact_launches": st["compact_launches"], "prep_seconds": round(t_prep, 2)}
+        if P == 64 and nphase == 0:
+            try:
+                rec["exec_bufs_host"] = exec_bufs_host(plan, bufs, x, P, k, algorithmic_bytes(len(x), len(ci)))
+            except Exception as e:
+                rec["exec_bufs_host"] = {"error": str(e)}
         if P == 64 and nphase == 0:  # the same plan collecting in a separate pass (collect_results)
             del plan
             plan = M.Plan(bufs, M.Plan.GATHER)

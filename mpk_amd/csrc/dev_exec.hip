@@ -250,6 +250,12 @@ struct mpkx_plan {
   DevArray<uint16_t> ci16;
   long rows = 0, nnz = 0, ghosts = 0;
   GraphCache graph;
+  // pinned host staging of the flat buffer for mpkx_exec_bufs (allocated on
+  // its first call): one DMA each way instead of pageable copies per rank
+  double *pinned = nullptr;
+  ~mpkx_plan() {
+    if (pinned) (void)hipHostFree(pinned);
+  }
 };
 
 extern "C" {
@@ -594,15 +600,24 @@ int mpkx_exec_bufs(mpkx_plan *pl, mpkx_buffers *const *bufs) {
   for (int p = 0; p < pl->npart; p++)
     if (!bufs[p] || !bufs[p]->vv_buf || bufs[p]->buf_count != pl->buf_count[p])
       return fail(MPKX_EINVAL, "exec_bufs: rank %d buffers do not match the plan", p);
+  if (!pl->pinned) MPKX_HIP(hipHostMalloc((void **)&pl->pinned, sizeof(double) * pl->flat, hipHostMallocDefault));
+  // inputs: every rank's rbuf[0] into the pinned image (host threads), then
+  // one DMA per rank
+  parallel_for(pl->npart, [&](long p) {
+    memcpy(pl->pinned + pl->base[p], bufs[p]->vv_buf, sizeof(double) * bufs[p]->rcount[0]);
+  });
   for (int p = 0; p < pl->npart; p++)
-    MPKX_HIP(hipMemcpy(pl->vv.p + pl->base[p], bufs[p]->vv_buf, sizeof(double) * bufs[p]->rcount[0],
-                       hipMemcpyHostToDevice));
+    MPKX_HIP(hipMemcpyAsync(pl->vv.p + pl->base[p], pl->pinned + pl->base[p], sizeof(double) * bufs[p]->rcount[0],
+                            hipMemcpyHostToDevice, nullptr));
   int rc = run_phases(pl, nullptr);
   if (rc) return rc;
-  MPKX_HIP(hipDeviceSynchronize());
-  for (int p = 0; p < pl->npart; p++)
-    MPKX_HIP(hipMemcpy(bufs[p]->vv_buf, pl->vv.p + pl->base[p], sizeof(double) * pl->buf_count[p],
-                       hipMemcpyDeviceToHost));
+  // outputs: the whole flat buffer in one DMA, then every rank's vv_buf
+  // from the pinned image (host threads)
+  MPKX_HIP(hipMemcpyAsync(pl->pinned, pl->vv.p, sizeof(double) * pl->flat, hipMemcpyDeviceToHost, nullptr));
+  MPKX_HIP(hipStreamSynchronize(nullptr));
+  parallel_for(pl->npart, [&](long p) {
+    memcpy(bufs[p]->vv_buf, pl->pinned + pl->base[p], sizeof(double) * pl->buf_count[p]);
+  });
   return MPKX_OK;
 }
 
