@@ -107,8 +107,29 @@ void skirt_table(Ctx &c, std::vector<uint64_t> &keys) {
       }
 }
 
+// Sorted, duplicates removed.  LSD radix sort on 11-bit digits over the
+// bits in use (the pair s*np+t above bit 32, the vv index below): at 128^3
+// with 64 partitions a table holds about 18M keys, which std::sort took
+// most of the prep time to order.
 void sort_unique(std::vector<uint64_t> &k) {
-  std::sort(k.begin(), k.end());
+  uint64_t used = 0;
+  for (uint64_t x : k) used |= x;
+  constexpr int D = 11;
+  constexpr size_t NB = (size_t)1 << D;
+  std::vector<uint64_t> tmp(k.size());
+  std::vector<size_t> cnt(NB);
+  for (int shift = 0; shift < 64 && (used >> shift) != 0; shift += D) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (uint64_t x : k) cnt[(x >> shift) & (NB - 1)]++;
+    size_t sum = 0;
+    for (size_t b = 0; b < NB; b++) {
+      const size_t c = cnt[b];
+      cnt[b] = sum;
+      sum += c;
+    }
+    for (uint64_t x : k) tmp[cnt[(x >> shift) & (NB - 1)]++] = x;
+    k.swap(tmp);
+  }
   k.erase(std::unique(k.begin(), k.end()), k.end());
 }
 

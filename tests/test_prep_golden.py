@@ -146,6 +146,25 @@ def test_fill_buffers_graph_partitions_match_oracle():
         assert bufs.rank_bytes(r) == blobs[r]
 
 
+@pytest.mark.parametrize("nphase", [0, 1])
+def test_fill_buffers_cube24_match_oracle(nphase):
+    """A larger case (24^3, 8 partitions, k = 6): every rank's buffers
+    byte-equal to the oracle's (the communication tables hold tens of
+    thousands of keys over all digit passes of the table sort)."""
+    N, nlevel, npart = 24, 6, 8
+    rp, ci, va = M.gen2("cube", N, -1.0)
+    parts = [M.partition_grid([N] * 3, [2, 2, 2], [6 * ph] * 3) for ph in range(max(nphase, 1))]
+    levels, sk, bufs, relab = M.prepare(rp, ci, va, nlevel, nphase, parts)
+    ol, prev = [], None
+    for ph in range(nphase):
+        prev = O.levels(rp, ci, parts[ph], prev)
+        ol.append(prev)
+    sl = O.skirt_filter(O.skirt_dist(rp, ci, parts[0]), nlevel, ol[-1] if nphase else None)
+    blobs, _ = O.fill_buffers(rp, ci, va, nlevel, nphase, parts, ol, sl)
+    for r in range(npart):
+        assert bufs.rank_bytes(r) == blobs[r]
+
+
 def test_prep_mpk_task_lists_match_oracle():
     for case in ["m5p8_4_4_0", "m5p64_4_4_2", "cube16_8_8_1", "m9p16_8_6_2"]:
         d, rp, ci, va, parts, levs = _ref_levels(case)
