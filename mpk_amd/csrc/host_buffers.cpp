@@ -251,12 +251,11 @@ int mpkx_fill_buffers(int n, const int *ptr, const int *col, const double *val, 
       cnt1[ph].assign((size_t)npart * npart, 0);
       for (uint64_t k : keys) cnt1[ph][k >> 32]++;
     }
-    std::vector<uint64_t> keys;
-    if (nphase == 0) add_init(c, keys);
-    skirt_table(c, keys);
-    sort_unique(keys);
-    cnt1[nphase].assign((size_t)npart * npart, 0);
-    for (uint64_t k : keys) cnt1[nphase][k >> 32]++;
+    // The final (skirt) table is built once, in pass 2: the phase tables
+    // never read values computed in the final phase (a schedule that does
+    // fails with c.err), and the skirt table sets the same store_part
+    // entries in the same order whether or not it ran before, so a sizing
+    // run of it would only repeat pass 2's keys.
   } catch (const std::bad_alloc &) {
     return fail(MPKX_ENOMEM, "fill_buffers: pass 1 allocation");
   }
@@ -284,7 +283,7 @@ int mpkx_fill_buffers(int n, const int *ptr, const int *col, const double *val, 
     return fail(MPKX_ENOMEM, "fill_buffers: pass 2 allocation");
   }
   if (c.err) return fail(MPKX_ESTRUCT, "fill_buffers: a source value is never stored (schedule inconsistent)");
-  for (int ph = 0; ph <= nphase; ph++)
+  for (int ph = 0; ph < nphase; ph++)
     if (comm[ph].cnt != cnt1[ph])
       return fail(MPKX_ESTRUCT, "fill_buffers: phase %d communication differs between passes", ph);
 
