@@ -21,10 +21,8 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <memory>
 #include <new>
-#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -37,14 +35,6 @@ using namespace mpkx;
 namespace {
 
 enum CommType { kMcol = 0, kInitIdcs = 1, kFinished = 2 };
-
-struct SrcType {
-  int src_mbuf_idx;
-  int type;
-  bool operator<(const SrcType &o) const {
-    return std::tie(src_mbuf_idx, type) < std::tie(o.src_mbuf_idx, o.type);
-  }
-};
 
 // PdmpkBuffers (pdmpk/pdmpk_buffers.cc): partial-edge bits, partitions and
 // levels of every vertex.
@@ -142,7 +132,22 @@ class Patterns {
   std::vector<int> sp_part_, sp_idx_;  // store_part[(idx, lvl)] = (part, mbuf idx)
   std::vector<int> home_;
   std::vector<std::vector<int>> history_;
-  std::map<std::pair<int, int>, std::map<SrcType, std::set<int>>> comm_dict_;
+  // The batch's communication (the reference's comm_dict: a std::map of
+  // (src, tgt) to a std::map of (src_mbuf_idx, type) to a std::set of ints) as flat
+  // records; sorted by (src, tgt, src_mbuf_idx, type, value) with exact
+  // duplicates removed they group and iterate in the nested containers'
+  // order, without a tree node per insert.
+  struct CommRec {
+    int src, tgt, m, type, v;
+    bool operator<(const CommRec &o) const {
+      return std::tie(src, tgt, m, type, v) < std::tie(o.src, o.tgt, o.m, o.type, o.v);
+    }
+    bool operator==(const CommRec &o) const {
+      return src == o.src && tgt == o.tgt && m == o.m && type == o.type && v == o.v;
+    }
+  };
+  std::vector<CommRec> comm_;
+  void comm_add(int src, int tgt, int m, int type, int v) { comm_.push_back({src, tgt, m, type, v}); }
 
   size_t key(int idx, int lvl) const { return (size_t)idx * (nl_ + 1) + lvl; }
   bool sp_has(int idx, int lvl) const { return sp_part_[key(idx, lvl)] >= 0; }
@@ -167,7 +172,7 @@ class Patterns {
     if (!sp_has(idx, nl_)) return;
     int tgt = home_[idx];
     int src = sp_part_[key(idx, nl_)], m = sp_idx_[key(idx, nl_)];
-    if (src != tgt) comm_dict_[{src, tgt}][{m, kFinished}].insert(idx);
+    if (src != tgt) comm_add(src, tgt, m, kFinished, idx);
   }
 
   void pre_batch() {  // :481-495 PreBatch + Buffers::PreBatch (buffers.cc:30-34)
@@ -196,9 +201,21 @@ class Patterns {
   }
 
   void post_batch(short lbelow) {  // :520-545 PostBatch
-    for (auto &it : comm_dict_) {  // UpdateMpiCountBuffers, :614-625
-      int src = it.first.first, tgt = it.first.second;
-      int size = (int)it.second.size();
+    std::sort(comm_.begin(), comm_.end());
+    comm_.erase(std::unique(comm_.begin(), comm_.end()), comm_.end());
+    // (src, tgt) groups [g0, g1): each distinct (m, type) is one value sent
+    std::vector<std::pair<size_t, size_t>> groups;
+    for (size_t a = 0; a < comm_.size();) {
+      size_t b = a;
+      while (b < comm_.size() && comm_[b].src == comm_[a].src && comm_[b].tgt == comm_[a].tgt) b++;
+      groups.push_back({a, b});
+      a = b;
+    }
+    for (auto &g : groups) {  // UpdateMpiCountBuffers, :614-625
+      int src = comm_[g.first].src, tgt = comm_[g.first].tgt;
+      int size = 0;
+      for (size_t r = g.first; r < g.second; r++)
+        if (r == g.first || comm_[r].m != comm_[r - 1].m || comm_[r].type != comm_[r - 1].type) size++;
       bufs_[tgt].recvcounts[(size_t)batch_ * np_ + src] += size;
       bufs_[src].sendcounts[(size_t)batch_ * np_ + tgt] += size;
     }
@@ -216,11 +233,11 @@ class Patterns {
       b.mbuf_idx += (int)rbuf_size(b, batch_);
     }
     std::vector<char> needs_wait(np_, 0);
-    for (auto &it : comm_dict_) {
-      needs_wait[it.first.second] = 1;
-      proc_comm_dict(it.first.first, it.first.second, it.second);
+    for (auto &g : groups) {
+      needs_wait[comm_[g.first].tgt] = 1;
+      proc_comm_dict(g.first, g.second);
     }
-    comm_dict_.clear();
+    comm_.clear();
     for (int i = 0; i < np_; i++) {
       auto &pd = bufs_[i].pds.back();
       update_bottom(pd, lbelow);
@@ -232,26 +249,32 @@ class Patterns {
     }
   }
 
-  void proc_comm_dict(int src, int tgt, const std::map<SrcType, std::set<int>> &bp) {  // :627-664
+  // One (src, tgt) group of records [g0, g1), sorted: every distinct
+  // (src_mbuf_idx, type) is one sent value; its records' v ascending.
+  void proc_comm_dict(size_t g0, size_t g1) {  // :627-664
+    const int src = comm_[g0].src, tgt = comm_[g0].tgt;
     PdBuffers &sb = bufs_[src];
     PdBuffers &tb = bufs_[tgt];
     size_t send_base = sb.sbuf_begin[batch_] + sb.sdispls[(size_t)batch_ * np_ + tgt];
     size_t recv_base = tb.mbuf_begin[batch_] + tb.mptr.size() - tb.mptr_begin[batch_] +
                        tb.rdispls[(size_t)batch_ * np_ + src];
     size_t idx = 0;
-    for (auto &e : bp) {
+    for (size_t e0 = g0; e0 < g1;) {
+      size_t e1 = e0;
+      while (e1 < g1 && comm_[e1].m == comm_[e0].m && comm_[e1].type == comm_[e0].type) e1++;
       int src_idx = (int)(recv_base + idx);
-      if (e.first.type == kMcol) {
-        for (int t : e.second) tb.mcol[t] = src_idx;
-      } else if (e.first.type == kInitIdcs) {
-        tb.init_idcs.push_back({src_idx, *e.second.begin()});
+      if (comm_[e0].type == kMcol) {
+        for (size_t r = e0; r < e1; r++) tb.mcol[comm_[r].v] = src_idx;
+      } else if (comm_[e0].type == kInitIdcs) {
+        tb.init_idcs.push_back({src_idx, comm_[e0].v});
       } else {
-        int orig = *e.second.begin();
+        int orig = comm_[e0].v;
         sp_part_[key(orig, nl_)] = tgt;
         sp_idx_[key(orig, nl_)] = src_idx;
       }
-      sb.sbuf_idcs[send_base + idx] = e.first.src_mbuf_idx;
+      sb.sbuf_idcs[send_base + idx] = comm_[e0].m;
       idx++;
+      e0 = e1;
     }
   }
 
@@ -260,7 +283,7 @@ class Patterns {
     int tp = st_.partitions[idx];
     int ti = bufs_[tp].mbuf_idx;
     if (sp != tp)
-      comm_dict_[{sp, tp}][{si, kInitIdcs}].insert(ti);
+      comm_add(sp, tp, si, kInitIdcs, ti);
     else
       bufs_[tp].init_idcs.push_back({si, ti});
   }
@@ -276,7 +299,7 @@ class Patterns {
       b.mcol.push_back(si);
     } else {
       int ti = (int)b.mcol.size();
-      comm_dict_[{sp, cp}][{si, kMcol}].insert(ti);
+      comm_add(sp, cp, si, kMcol, ti);
       b.mcol.push_back(-1);
     }
   }
@@ -315,7 +338,12 @@ class Patterns {
   // FindLabelPermutation (:463-479) on a copy of the state.
   void optimize_labels(short min_level) {
     PdState cnt = st_;
-    std::map<std::pair<int, int>, std::set<int>> table;
+    // (s, t) -> the distinct source slots: only the counts are used, so
+    // keys (s*np + t, slot) sorted and made unique
+    std::vector<uint64_t> table;
+    auto tab = [&](int s, int t, int slot) {
+      table.push_back(((uint64_t)((long)s * np_ + t) << 32) | (uint32_t)slot);
+    };
     bool active = true;
     for (int lbelow = min_level; active && lbelow < nl_; lbelow++) {
       active = false;
@@ -329,24 +357,21 @@ class Patterns {
             cnt.partials[t] = 1;
             int j = col_[t];
             if (sp_has(j, lbelow)) {
-              table[{sp_part_[key(j, lbelow)], tp}].insert(sp_idx_[key(j, lbelow)]);
+              tab(sp_part_[key(j, lbelow)], tp, sp_idx_[key(j, lbelow)]);
               ret = true;
             }
           }
         if (ret) {
-          if (send_partial)
-            table[{sp_part_[key(idx, lbelow + 1)], tp}].insert(sp_idx_[key(idx, lbelow + 1)]);
+          if (send_partial) tab(sp_part_[key(idx, lbelow + 1)], tp, sp_idx_[key(idx, lbelow + 1)]);
           cnt.IncLevel(idx);
           active = true;
         }
       }
     }
-    std::vector<int> sums((size_t)np_ * np_);
-    for (int s = 0; s < np_; s++)
-      for (int t = 0; t < np_; t++) {
-        auto it = table.find({s, t});
-        sums[(size_t)s * np_ + t] = it == table.end() ? 0 : (int)it->second.size();
-      }
+    std::sort(table.begin(), table.end());
+    table.erase(std::unique(table.begin(), table.end()), table.end());
+    std::vector<int> sums((size_t)np_ * np_, 0);
+    for (uint64_t k : table) sums[k >> 32]++;
     int mn = *std::min_element(sums.begin(), sums.end());
     for (auto &v : sums) v -= mn;
     std::vector<int> perm(np_);
