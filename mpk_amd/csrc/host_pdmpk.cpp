@@ -149,7 +149,8 @@ class Patterns {
   std::vector<CommRec> comm_;
   void comm_add(int src, int tgt, int m, int type, int v) { comm_.push_back({src, tgt, m, type, v}); }
 
-  size_t key(int idx, int lvl) const { return (size_t)idx * (nl_ + 1) + lvl; }
+  // level-major: the neighbours of a vertex at one level are close in memory
+  size_t key(int idx, int lvl) const { return (size_t)lvl * n_ + idx; }
   bool sp_has(int idx, int lvl) const { return sp_part_[key(idx, lvl)] >= 0; }
 
   static void update_bottom(PdPhaseDescriptor &pd, short l) {  // phase_descriptor.cc:10-15
